@@ -1,0 +1,536 @@
+// C ABI of the engine: registry, launch schedule, device-buffer management
+// and the forward entry points.  Host code only; kernels live in the
+// per-stencil translation units built from fwd_inst.hip.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/an5d.h"
+#include "engine.hpp"
+#include "internal.hpp"
+#include "stencil_defs.hpp"
+
+namespace an5d {
+
+// ---- errors -----------------------------------------------------------------
+static thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(AN5D_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- registry -----------------------------------------------------------------
+#define AN5D_DECL_REG(id, ident, dim, rad, fpp, n, div, lin) \
+  void register_variants_##id##_0(std::vector<Variant>&);     \
+  void register_variants_##id##_1(std::vector<Variant>&);
+AN5D_STENCILS(AN5D_DECL_REG)
+
+std::vector<Variant>& registry() {
+  static std::vector<Variant> v = [] {
+    std::vector<Variant> out;
+#define AN5D_CALL_REG(id, ident, dim, rad, fpp, n, div, lin) \
+  register_variants_##id##_0(out);                            \
+  register_variants_##id##_1(out);
+    AN5D_STENCILS(AN5D_CALL_REG)
+    return out;
+  }();
+  return v;
+}
+
+std::vector<const Variant*> variants_of(int stencil, int dtype) {
+  std::vector<const Variant*> out;
+  for (const Variant& v : registry())
+    if (v.stencil == stencil && v.dtype == dtype) out.push_back(&v);
+  return out;
+}
+
+const StencilRT& stencil_rt(int id) {
+  static const std::vector<StencilRT> table = [] {
+    std::vector<StencilRT> t;
+#define AN5D_RT(id_, ident, dim_, rad_, fpp_, n_, div_, lin_)                       \
+  {                                                                                  \
+    StencilRT s;                                                                     \
+    s.name = #ident;                                                                 \
+    for (char& ch : s.name) if (ch == '_') ch = '-';                                 \
+    s.dim = dim_; s.rad = rad_; s.fpp = fpp_; s.ntaps = n_; s.divisor = div_;        \
+    s.linear = lin_;                                                                 \
+    for (int q = 0; q < n_; q++) s.taps.push_back(Def<id_>::taps[q]);                \
+    t.push_back(s);                                                                  \
+  }
+    AN5D_STENCILS(AN5D_RT)
+    return t;
+  }();
+  return table[id];
+}
+
+// ---- geometry ---------------------------------------------------------------
+bool make_geom(const an5d_problem* p, Geom& g, std::string& err) {
+  if (!p) { err = "null problem"; return false; }
+  if (p->stencil < 0 || p->stencil >= kNumStencils) { err = "unknown stencil id"; return false; }
+  if (p->dtype != AN5D_FLOAT32 && p->dtype != AN5D_FLOAT64) { err = "bad dtype"; return false; }
+  const StencilRT& s = stencil_rt(p->stencil);
+  const int nd = s.dim;
+  for (int d = 0; d < nd; d++)
+    if (p->n[d] < 1) { err = "interior extents must be >= 1"; return false; }
+  g.r = s.rad;
+  g.dim = nd;
+  g.D0 = p->n[0] + 2 * s.rad;
+  if (nd == 2) { g.D1 = 1; g.D2 = p->n[1] + 2 * s.rad; }
+  else { g.D1 = p->n[1] + 2 * s.rad; g.D2 = p->n[2] + 2 * s.rad; }
+  if (g.D0 > (1LL << 30) || g.D1 > (1LL << 30) || g.D2 > (1LL << 30)) {
+    err = "extent too large";
+    return false;
+  }
+  g.area = g.D0 * g.D1 * g.D2;
+  return true;
+}
+
+an5d_layout reference_layout(const Geom& g) {
+  an5d_layout L;
+  L.s1 = g.D2;
+  L.s0 = g.D1 * g.D2;
+  L.buf_stride = g.area;
+  return L;
+}
+
+an5d_layout padded_layout(const Geom& g) {
+  an5d_layout L;
+  const long long pitch = (g.D2 + 63) / 64 * 64;  // 256-byte (FP32) rows
+  L.s1 = pitch;
+  L.s0 = g.D1 * pitch;
+  L.buf_stride = g.D0 * L.s0;
+  return L;
+}
+
+// ---- launch planning ----------------------------------------------------------
+static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+int default_sl(const Variant& v, int bt, const Geom& g, long long ntk, long long ntj) {
+  const long long n0 = g.D0 - 2 * g.r;
+  long long tiles = ntk * ntj * (v.kind == kSweep && g.dim == 3 ? v.nw : 1);
+  long long nseg = cdiv(8192, tiles > 0 ? tiles : 1);
+  long long sl = cdiv(n0, nseg);
+  long long minsl = 8LL * bt * g.r;
+  if (sl < minsl) sl = minsl;
+  if (sl > n0) sl = n0;
+  if (sl < 1) sl = 1;
+  return (int)sl;
+}
+
+bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp,
+                 std::string& err) {
+  const int R = g.r;
+  const long long n0 = g.D0 - 2 * R, nj = g.D1 - (g.dim == 3 ? 2 * R : 0), nk = g.D2 - 2 * R;
+  LaunchArgs& a = lp.args;
+  a.D0 = (int)g.D0; a.D1 = (int)g.D1; a.D2 = (int)g.D2;
+  lp.fn = (bt >= 1 && bt <= v.btmax) ? v.fn[bt] : nullptr;
+  if (!lp.fn) { err = "no kernel for bt=" + std::to_string(bt) + " in this variant"; return false; }
+  lp.lds = v.lds;
+  if (v.kind == kDirect) {
+    const long long total = cdiv(nk, v.ck) * nj * n0;
+    lp.block = dim3(256);
+    lp.grid = dim3((unsigned)cdiv(total, 256));
+    a.ntk = a.ntj = a.nseg = a.sl = a.SK = a.SJ = 0;
+    if (cdiv(total, 256) >= (1LL << 31)) { err = "grid too large"; return false; }
+    return true;
+  }
+  const long long SK = 64LL * v.ck - 2LL * bt * R;
+  if (SK <= 0) { err = "bt too deep for this variant's tile width"; return false; }
+  const long long ntk = cdiv(nk, SK);
+  long long SJ = 1, ntj = 1;
+  if (g.dim == 3) {
+    SJ = (long long)v.nw * v.cj - 2LL * bt * R;
+    if (SJ <= 0) { err = "bt too deep for this variant's tile height"; return false; }
+    ntj = cdiv(nj, SJ);
+  }
+  if (sl <= 0) sl = default_sl(v, bt, g, ntk, ntj);
+  const long long nseg = cdiv(n0, sl);
+  a.ntk = (int)ntk; a.ntj = (int)ntj; a.nseg = (int)nseg; a.sl = sl;
+  a.SK = (int)SK; a.SJ = (int)SJ;
+  lp.block = dim3(64 * v.nw);
+  long long blocks = (g.dim == 3) ? ntk * ntj * nseg : cdiv(ntk * nseg, v.nw);
+  if (blocks >= (1LL << 31)) { err = "grid too large"; return false; }
+  lp.grid = dim3((unsigned)blocks);
+  return true;
+}
+
+// ---- coefficient buffers -------------------------------------------------------
+// Host image of the kernel's coefficient array: ntaps values (T), then the
+// divisor; for gradient2d {eps, 0, 0}.
+bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,
+                 double& gscale, std::string& err) {
+  const StencilRT& s = stencil_rt(p->stencil);
+  const bool f64 = p->dtype == AN5D_FLOAT64;
+  const int nt = s.linear ? s.ntaps : 1;
+  std::vector<double> vals(nt + 1, 0.0);
+  gscale = 1.0;
+  has_div = 0;
+  if (s.linear) {
+    for (int q = 0; q < s.ntaps; q++)
+      vals[q] = p->coefs ? p->coefs[q] : (double)s.taps[q].coef;
+    if (p->divisor < 0) {
+      has_div = s.divisor != 0;
+      vals[nt] = s.divisor;
+    } else if (p->divisor > 0) {
+      has_div = 1;
+      vals[nt] = p->divisor;
+    }
+  } else {
+    vals[0] = p->coefs ? p->coefs[0] : (double)0.0001f;
+    gscale = p->coefs ? p->coefs[1] : 1.0;
+    if (!(gscale == gscale)) { err = "scale is NaN"; return false; }
+  }
+  const size_t esz = f64 ? 8 : 4;
+  bytes.resize((nt + 1) * esz);
+  for (int q = 0; q <= nt; q++) {
+    if (f64) {
+      double v = vals[q];
+      memcpy(&bytes[q * 8], &v, 8);
+    } else {
+      float v = (float)vals[q];
+      memcpy(&bytes[q * 4], &v, 4);
+    }
+  }
+  return true;
+}
+
+struct DeviceCache {
+  std::mutex mu;
+  std::map<std::pair<int, std::string>, void*> coefs;  // (device, bytes) -> device copy
+  std::map<int, std::pair<void*, size_t>> grids;       // device -> staging grid buffer
+};
+static DeviceCache& cache() {
+  static DeviceCache* c = new DeviceCache();
+  return *c;
+}
+
+int device_coefs(const std::vector<unsigned char>& bytes, const void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  DeviceCache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  std::string key(bytes.begin(), bytes.end());
+  auto it = c.coefs.find({dev, key});
+  if (it != c.coefs.end()) {
+    *out = it->second;
+    return AN5D_OK;
+  }
+  void* d = nullptr;
+  e = hipMalloc(&d, bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(coefficients)");
+  e = hipMemcpy(d, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { (void)hipFree(d); return hip_fail(e, "hipMemcpy(coefficients)"); }
+  c.coefs[{dev, key}] = d;
+  *out = d;
+  return AN5D_OK;
+}
+
+int device_grid(size_t bytes, void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  DeviceCache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto& slot = c.grids[dev];
+  if (slot.second < bytes) {
+    if (slot.first) (void)hipFree(slot.first);
+    slot = {nullptr, 0};
+    void* d = nullptr;
+    e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) return fail(AN5D_E_ALLOC, std::string("hipMalloc(grid): ") + hipGetErrorString(e));
+    slot = {d, bytes};
+  }
+  *out = slot.first;
+  return AN5D_OK;
+}
+
+// ---- variant / bt resolution ---------------------------------------------------
+int default_bt(const StencilRT& s, int dtype) {
+  if (s.dim == 2) return s.rad == 1 ? 4 : 2;
+  return s.rad == 1 ? 2 : 1;
+}
+
+int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout, int* btout) {
+  auto vs = variants_of(p->stencil, p->dtype);
+  if (vs.empty()) return fail(AN5D_E_UNSUPPORTED, "no kernels compiled for this stencil/dtype");
+  const StencilRT& s = stencil_rt(p->stencil);
+  int want_bt = cfg ? cfg->bt : 0;
+  int want_v = cfg ? cfg->variant : -1;
+  if (want_v >= (int)vs.size()) return fail(AN5D_E_INVALID, "variant index out of range");
+  if (want_bt < 0 || want_bt > kMaxBT) return fail(AN5D_E_INVALID, "bt out of range");
+  const Variant* v = nullptr;
+  if (want_v >= 0) {
+    v = vs[want_v];
+  } else {
+    TunedChoice tc;
+    if (tuned_lookup(p, &tc) && tc.variant < (int)vs.size()) {
+      v = vs[tc.variant];
+      if (want_bt == 0) want_bt = tc.bt;
+    } else {
+      v = vs[0];  // first sweep shape, or the direct kernel when none exists
+    }
+  }
+  int bt = want_bt ? want_bt : default_bt(s, p->dtype);
+  if (bt > v->btmax) {
+    if (want_bt) return fail(AN5D_E_UNSUPPORTED, "bt=" + std::to_string(bt) + " exceeds the variant's btmax=" + std::to_string(v->btmax));
+    bt = v->btmax;
+  }
+  *vout = v;
+  *btout = bt;
+  return AN5D_OK;
+}
+
+}  // namespace an5d
+
+using namespace an5d;
+
+// ---- schedule -------------------------------------------------------------------
+extern "C" int an5d_schedule(int T, int bt, int* steps, int max_launches) {
+  if (T < 0 || bt < 1) return fail(AN5D_E_INVALID, "schedule needs T >= 0 and bt >= 1");
+  std::vector<int> out;
+  const int full = T / bt, r = T % bt;
+  const int nat = (T + bt - 1) / bt;  // launches of the plain split
+  const bool mismatch = (T % 2) != (nat % 2);
+  int main_count = full;
+  if (mismatch && r < 2) main_count -= 1;  // __c0Padr (host.cu:72)
+  for (int c = 0; c < main_count; c++) out.push_back(bt);
+  if (mismatch) {
+    if (r == 0) {
+      out.push_back((bt + 1) / 2);
+      out.push_back(bt / 2);
+    } else if (r == 1) {
+      const int b = (bt + 1) / 3;
+      out.push_back(bt + 1 - 2 * b);
+      out.push_back(b);
+      out.push_back(b);
+    } else {
+      out.push_back((r + 1) / 2);
+      out.push_back(r / 2);
+    }
+  } else if (r) {
+    out.push_back(r);
+  }
+  if (steps) {
+    if ((int)out.size() > max_launches) return fail(AN5D_E_INVALID, "steps[] too small");
+    for (size_t c = 0; c < out.size(); c++) steps[c] = out[c];
+  }
+  return (int)out.size();
+}
+
+// ---- registry queries ---------------------------------------------------------
+extern "C" int an5d_stencil_count(void) { return kNumStencils; }
+
+extern "C" int an5d_stencil_id(const char* name) {
+  if (!name) return fail(AN5D_E_INVALID, "null name");
+  for (int i = 0; i < kNumStencils; i++)
+    if (stencil_rt(i).name == name) return i;
+  return fail(AN5D_E_INVALID, std::string("unknown stencil: ") + name);
+}
+
+extern "C" int an5d_stencil_get_info(int id, an5d_stencil_info* out) {
+  if (id < 0 || id >= kNumStencils || !out) return fail(AN5D_E_INVALID, "bad stencil id");
+  const StencilRT& s = stencil_rt(id);
+  memset(out, 0, sizeof(*out));
+  snprintf(out->name, sizeof(out->name), "%s", s.name.c_str());
+  out->dim = s.dim; out->rad = s.rad; out->fpp = s.fpp; out->ntaps = s.ntaps;
+  out->divisor = s.divisor; out->linear = s.linear;
+  return AN5D_OK;
+}
+
+extern "C" int an5d_stencil_taps(int id, int* offsets, double* coefs, int max_taps) {
+  if (id < 0 || id >= kNumStencils) return fail(AN5D_E_INVALID, "bad stencil id");
+  const StencilRT& s = stencil_rt(id);
+  if (max_taps < s.ntaps) return fail(AN5D_E_INVALID, "buffer too small");
+  for (int q = 0; q < s.ntaps; q++) {
+    if (offsets) {
+      offsets[4 * q] = s.taps[q].sign;
+      offsets[4 * q + 1] = s.taps[q].d0;
+      offsets[4 * q + 2] = s.taps[q].d1;
+      offsets[4 * q + 3] = s.taps[q].d2;
+    }
+    if (coefs) coefs[q] = s.taps[q].coef;
+  }
+  return s.ntaps;
+}
+
+extern "C" int an5d_variant_count(int stencil, int dtype) {
+  if (stencil < 0 || stencil >= kNumStencils) return fail(AN5D_E_INVALID, "bad stencil id");
+  return (int)variants_of(stencil, dtype).size();
+}
+
+extern "C" int an5d_variant_get_info(int stencil, int dtype, int variant, an5d_variant_info* out) {
+  auto vs = variants_of(stencil, dtype);
+  if (variant < 0 || variant >= (int)vs.size() || !out) return fail(AN5D_E_INVALID, "bad variant");
+  const Variant* v = vs[variant];
+  out->kind = v->kind; out->ck = v->ck; out->cj = v->cj; out->nw = v->nw;
+  out->btmax = v->btmax; out->lds_bytes = (long long)v->lds;
+  return AN5D_OK;
+}
+
+// ---- forward ------------------------------------------------------------------------
+namespace an5d {
+
+int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
+                 const an5d_config* cfg, int steps_only, int c0_first, hipStream_t st) {
+  Geom g;
+  std::string err;
+  if (!make_geom(p, g, err)) return fail(AN5D_E_INVALID, err);
+  if (!dA) return fail(AN5D_E_INVALID, "null device pointer");
+  an5d_layout L = layout ? *layout : reference_layout(g);
+  if (L.s1 < g.D2 || (g.dim == 3 && L.s0 < g.D1 * L.s1) || (g.dim == 2 && L.s0 < g.D2) ||
+      L.buf_stride < g.D0 * L.s0)
+    return fail(AN5D_E_INVALID, "layout strides smaller than the grid");
+  const Variant* v = nullptr;
+  int bt = 0;
+  int rc = resolve(p, cfg, &v, &bt);
+  if (rc) return rc;
+  std::vector<unsigned char> cbytes;
+  int has_div = 0;
+  double gscale = 1.0;
+  if (!build_coefs(p, cbytes, has_div, gscale, err)) return fail(AN5D_E_INVALID, err);
+  const void* dcoef = nullptr;
+  rc = device_coefs(cbytes, &dcoef);
+  if (rc) return rc;
+  const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
+  std::vector<int> steps;
+  if (steps_only > 0) {
+    steps.push_back(steps_only);
+  } else {
+    int n = an5d_schedule(p->timesteps, v->kind == kDirect ? 1 : bt, nullptr, 0);
+    if (n < 0) return n;
+    steps.resize(n);
+    an5d_schedule(p->timesteps, v->kind == kDirect ? 1 : bt, steps.data(), n);
+  }
+  const int sl = cfg ? cfg->sl : 0;
+  int c0 = c0_first;
+  for (int k : steps) {
+    LaunchPlan lp;
+    if (!plan_launch(*v, k, g, sl, lp, err)) return fail(AN5D_E_UNSUPPORTED, err);
+    char* base = static_cast<char*>(dA);
+    lp.args.src = base + (size_t)(c0 % 2) * L.buf_stride * esz;
+    lp.args.dst = base + (size_t)((c0 + 1) % 2) * L.buf_stride * esz;
+    lp.args.coef = dcoef;
+    lp.args.gscale = gscale;
+    lp.args.has_div = has_div;
+    lp.args.s0 = L.s0;
+    lp.args.s1 = L.s1;
+    hipError_t e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    c0++;
+  }
+  return AN5D_OK;
+}
+
+double sb_time_ms() {
+  struct timespec tp;
+  clock_gettime(CLOCK_MONOTONIC_RAW, &tp);
+  return (tp.tv_sec + tp.tv_nsec * 1.0e-9) * 1.0e3;
+}
+
+}  // namespace an5d
+
+extern "C" int an5d_forward_device(const an5d_problem* p, void* dA, const an5d_layout* layout,
+                                   const an5d_config* cfg, void* stream) {
+  if (!p) return fail(AN5D_E_INVALID, "null problem");
+  if (p->timesteps < 0) return fail(AN5D_E_INVALID, "timesteps must be >= 0");
+  if (p->timesteps == 0) return AN5D_OK;
+  return run_schedule(p, dA, layout, cfg, 0, 0, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int an5d_launch(const an5d_problem* p, void* dA, const an5d_layout* layout,
+                           const an5d_config* cfg, int steps, int c0, void* stream) {
+  if (steps < 1 || c0 < 0) return fail(AN5D_E_INVALID, "steps must be >= 1 and c0 >= 0");
+  return run_schedule(p, dA, layout, cfg, steps, c0, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int an5d_forward_host(const an5d_problem* p, void* A, const an5d_config* cfg,
+                                 double* ms) {
+  Geom g;
+  std::string err;
+  if (!make_geom(p, g, err)) return fail(AN5D_E_INVALID, err);
+  if (!A) return fail(AN5D_E_INVALID, "null host array");
+  if (ms) *ms = 0.0;
+  if (p->timesteps < 0) return fail(AN5D_E_INVALID, "timesteps must be >= 0");
+  if (p->timesteps == 0) return AN5D_OK;
+  const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
+  an5d_layout L = padded_layout(g);
+  void* dA = nullptr;
+  int rc = device_grid((size_t)2 * L.buf_stride * esz, &dA);
+  if (rc) return rc;
+  const size_t rows = (size_t)g.D0 * g.D1;
+  hipError_t e;
+  for (int b = 0; b < 2; b++) {
+    e = hipMemcpy2D(static_cast<char*>(dA) + (size_t)b * L.buf_stride * esz, L.s1 * esz,
+                    static_cast<char*>(A) + (size_t)b * g.area * esz, g.D2 * esz, g.D2 * esz, rows,
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy2D H2D");
+  }
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+  const double t0 = sb_time_ms();
+  rc = run_schedule(p, dA, &L, cfg, 0, 0, nullptr);
+  if (rc) return rc;
+  e = hipDeviceSynchronize();
+  const double t1 = sb_time_ms();
+  if (e != hipSuccess) return hip_fail(e, "kernel execution");
+  for (int b = 0; b < 2; b++) {
+    e = hipMemcpy2D(static_cast<char*>(A) + (size_t)b * g.area * esz, g.D2 * esz,
+                    static_cast<char*>(dA) + (size_t)b * L.buf_stride * esz, L.s1 * esz, g.D2 * esz,
+                    rows, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy2D D2H");
+  }
+  if (ms) *ms = t1 - t0;
+  return AN5D_OK;
+}
+
+extern "C" int an5d_kernel_stencil(int stencil, int dtype, void* A1, int compsize, int timestep,
+                                   int scop, double* ms) {
+  if (stencil < 0 || stencil >= kNumStencils) return fail(AN5D_E_INVALID, "bad stencil id");
+  const StencilRT& s = stencil_rt(stencil);
+  an5d_problem p;
+  memset(&p, 0, sizeof(p));
+  p.stencil = stencil;
+  p.dtype = dtype;
+  for (int d = 0; d < s.dim; d++) p.n[d] = compsize;
+  p.timesteps = timestep;
+  p.coefs = nullptr;
+  p.divisor = -1.0;
+  if (ms) *ms = 0.0;
+  // host.cu:17: nothing to do unless dimsize >= 2r+1 and timestep >= 1
+  if ((long long)compsize + 2 * s.rad < 2 * s.rad + 1 || timestep < 1) return AN5D_OK;
+  if (scop) {
+    // tiling overrides for run drivers: AN5D_BT, AN5D_VARIANT, AN5D_SL
+    an5d_config cfg{0, -1, 0};
+    if (const char* e = getenv("AN5D_BT")) cfg.bt = atoi(e);
+    if (const char* e = getenv("AN5D_VARIANT")) cfg.variant = atoi(e);
+    if (const char* e = getenv("AN5D_SL")) cfg.sl = atoi(e);
+    return an5d_forward_host(&p, A1, &cfg, ms);
+  }
+  return an5d_cpu_forward(&p, A1, 0, ms);
+}
+
+extern "C" int an5d_release(void) {
+  DeviceCache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (auto& kv : c.coefs) (void)hipFree(kv.second);
+  c.coefs.clear();
+  for (auto& kv : c.grids)
+    if (kv.second.first) (void)hipFree(kv.second.first);
+  c.grids.clear();
+  return AN5D_OK;
+}
+
+extern "C" const char* an5d_last_error(void) { return g_err.c_str(); }

@@ -1,0 +1,88 @@
+// Instantiates and registers the forward kernels of one stencil x dtype.
+// Built once per pair with -DAN5D_SID=<stencil id> -DAN5D_F64=<0|1>, so the
+// 42 translation units compile in parallel.
+#include "engine.hpp"
+#include "sweep.hpp"
+#include "variants.hpp"
+
+#ifndef AN5D_SID
+#error "AN5D_SID must be defined"
+#endif
+
+namespace an5d {
+namespace {
+
+using T = std::conditional_t<AN5D_F64, double, float>;
+constexpr int SID = AN5D_SID;
+
+template <typename U>
+SweepArgs<U> to_args(const LaunchArgs& a) {
+  SweepArgs<U> s;
+  s.src = static_cast<const U*>(a.src);
+  s.dst = static_cast<U*>(a.dst);
+  s.coef = static_cast<const U*>(a.coef);
+  s.gscale = a.gscale;
+  s.has_div = a.has_div;
+  s.D0 = a.D0; s.D1 = a.D1; s.D2 = a.D2;
+  s.s0 = a.s0; s.s1 = a.s1;
+  s.ntk = a.ntk; s.ntj = a.ntj; s.nseg = a.nseg; s.sl = a.sl;
+  s.SK = a.SK; s.SJ = a.SJ;
+  return s;
+}
+
+template <int BT, int CK, int CJ, int NW>
+hipError_t launch_sweep(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  if (lds > 65536) {
+    // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default once
+    static const hipError_t attr = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (attr != hipSuccess) return attr;
+  }
+  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW>), grid, block, lds, st, to_args<T>(a));
+  return hipGetLastError();
+}
+
+template <int CK>
+hipError_t launch_direct(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((direct_kernel<SID, T, CK>), grid, block, lds, st, to_args<T>(a));
+  return hipGetLastError();
+}
+
+template <int W>
+void add_sweep(std::vector<Variant>& out) {
+  constexpr Shape sh = sweep_shape<SID, T>(W);
+  if constexpr (sh.btmax > 0 && sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>() <= 160 * 1024) {
+    Variant v{};
+    v.stencil = SID;
+    v.dtype = AN5D_F64 ? kF64 : kF32;
+    v.kind = kSweep;
+    v.ck = sh.ck; v.cj = sh.cj; v.nw = sh.nw; v.btmax = sh.btmax;
+    v.lds = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>();
+    static_for<1, sh.btmax + 1>([&](auto bc) {
+      constexpr int BT = decltype(bc)::value;
+      v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw>;
+    });
+    out.push_back(v);
+  }
+}
+
+}  // namespace
+
+#define AN5D_CAT2(a, b, c) a##b##_##c
+#define AN5D_CAT(a, b, c) AN5D_CAT2(a, b, c)
+void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out) {
+  add_sweep<0>(out);
+  add_sweep<1>(out);
+  add_sweep<2>(out);
+  Variant d{};
+  d.stencil = SID;
+  d.dtype = AN5D_F64 ? kF64 : kF32;
+  d.kind = kDirect;
+  d.ck = direct_ck<SID, T>(); d.cj = 1; d.nw = 4; d.btmax = 1;
+  d.lds = 0;
+  d.fn[1] = &launch_direct<direct_ck<SID, T>()>;
+  out.push_back(d);
+}
+
+}  // namespace an5d

@@ -1,0 +1,56 @@
+// Declarations shared by the host translation units of liban5d.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/an5d.h"
+#include "engine.hpp"
+#include "stencil_defs.hpp"
+
+namespace an5d {
+
+struct StencilRT {
+  std::string name;
+  int dim, rad, fpp, ntaps, divisor;
+  bool linear;
+  std::vector<TapDef> taps;
+};
+
+struct Geom {
+  int dim, r;
+  long long D0, D1, D2, area;  // full extents (2D: D1 == 1)
+};
+
+struct LaunchPlan {
+  LaunchFn fn = nullptr;
+  LaunchArgs args{};
+  dim3 grid, block;
+  size_t lds = 0;
+};
+
+struct TunedChoice {
+  int variant, bt, sl;
+};
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+const StencilRT& stencil_rt(int id);
+std::vector<const Variant*> variants_of(int stencil, int dtype);
+bool make_geom(const an5d_problem* p, Geom& g, std::string& err);
+an5d_layout reference_layout(const Geom& g);
+an5d_layout padded_layout(const Geom& g);
+bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp,
+                 std::string& err);
+bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,
+                 double& gscale, std::string& err);
+int device_coefs(const std::vector<unsigned char>& bytes, const void** out);
+int device_grid(size_t bytes, void** out);
+int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
+                 const an5d_config* cfg, int steps_only, int c0_first, hipStream_t st);
+double sb_time_ms();
+
+// autotuner cache (autotune.cpp)
+bool tuned_lookup(const an5d_problem* p, TunedChoice* out);
+
+}  // namespace an5d

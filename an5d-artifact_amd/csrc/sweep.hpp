@@ -1,0 +1,433 @@
+// Temporally blocked stencil sweep for gfx950 (CDNA4), hand-written HIP.
+//
+// One launch advances the grid by BT fused timesteps: it reads buffer c0%2
+// and writes the interior of buffer (c0+1)%2, as the reference's kernel0_<k>
+// does (compiled/float/j2d5pt-256-10-128_kernel.cu:82-83), and leaves every
+// halo cell alone (Dirichlet: a halo value is the same at every step).
+//
+// Work decomposition (no relation to the reference's thread layout):
+//   * dim 0 (outermost) is streamed: a wave walks `sl` output planes of a
+//     segment plus a 2*BT*R warm-up, holding for every stage s < BT a ring
+//     of the last 2R+1 planes of u_s in VGPRs.  Per input plane it runs all
+//     BT stages back to back; stage s lags the input by s*R planes.
+//   * the fastest dim is owned lane-contiguously: each of the 64 lanes holds
+//     CK adjacent cells, so a wave spans 64*CK cells, loads them with one
+//     coalesced vector load per row and gets the R cells left/right of its
+//     run from the neighbouring lane with DPP wave_shr:1 / wave_shl:1 (no
+//     LDS, no barrier).  Lanes at the wave's two ends receive garbage, which
+//     spreads R cells per stage: a wave therefore produces 64*CK - 2*BT*R
+//     valid cells (overlapped tiling).
+//   * 3D only: the middle dim is split over the NW waves of a workgroup, CJ
+//     rows per lane.  Rows owned by another wave come through LDS (one
+//     double-buffered exchange and one barrier per stage that needs them).
+//     In 2D the NW waves of a workgroup are independent tiles.
+//
+// Exactness: each cell of each stage is produced by the same expression, in
+// the same term order, from the same operands as the C loop (no contraction:
+// the translation unit is built with -ffp-contract=off), so the result is
+// bitwise independent of BT, CK, CJ, NW and sl and equal to the CPU loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "stencil_defs.hpp"
+
+namespace an5d {
+
+template <typename T>
+struct SweepArgs {
+  const T* __restrict__ src;   // buffer c0 % 2
+  T* __restrict__ dst;         // buffer (c0 + 1) % 2
+  const T* __restrict__ coef;  // ntaps coefficients, then the divisor
+  double gscale;               // gradient2d: factor on the reciprocal-root term
+  int has_div;                 // divide the tap sum by coef[ntaps]
+  int D0, D1, D2;              // full extents incl. halo (2D: D1 == 1)
+  long long s0, s1;            // element strides of dim 0 and dim 1 (dim 2 is unit)
+  int ntk, ntj, nseg, sl;      // tiles along dim 2 / dim 1, segments along dim 0, segment length
+  int SK, SJ;                  // output cells per tile along dim 2 / dim 1
+};
+
+// ---- cross-lane moves ------------------------------------------------------
+// lane L receives lane L-1's value (wave_shr:1) / lane L+1's value (wave_shl:1)
+__device__ __forceinline__ int dpp_shr1(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_shl1(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float from_left(float v) {
+  return __builtin_bit_cast(float, dpp_shr1(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(float, dpp_shl1(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double from_left(double v) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = dpp_shr1((int)(b & 0xffffffffLL)), hi = dpp_shr1((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = dpp_shl1((int)(b & 0xffffffffLL)), hi = dpp_shl1((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ---- vector memory access with 4-byte alignment only -----------------------
+template <typename T, int N>
+struct Vec { T v[N]; };
+
+template <typename T, int N>
+__device__ __forceinline__ void load_run(T (&dst)[N], const T* p) {
+  Vec<T, N> t;
+  __builtin_memcpy(&t, p, sizeof(t));
+#pragma unroll
+  for (int i = 0; i < N; i++) dst[i] = t.v[i];
+}
+template <typename T, int N>
+__device__ __forceinline__ void store_run(T* p, const T (&src)[N]) {
+  Vec<T, N> t;
+#pragma unroll
+  for (int i = 0; i < N; i++) t.v[i] = src[i];
+  __builtin_memcpy(p, &t, sizeof(t));
+}
+
+// ---- one stencil evaluation, exact source order ---------------------------
+// Get::at<d0,d1,d2>() returns the stage input at that offset.
+template <class D, typename T, class Get>
+__device__ __forceinline__ T eval_point(const Get& get, const T* c, T div, bool has_div,
+                                        double gscale) {
+#pragma clang fp contract(off)
+  if constexpr (D::linear) {
+    T acc;
+    static_for<0, D::ntaps>([&](auto qc) {
+      constexpr int Q = decltype(qc)::value;
+      constexpr TapDef t = D::taps[Q];
+      const T v = get.template at<t.d0, t.d1, t.d2>();
+      if constexpr (Q == 0) acc = c[0] * v;
+      else if constexpr (t.sign > 0) acc = acc + c[Q] * v;
+      else acc = acc - c[Q] * v;
+    });
+    if (has_div) acc = acc / div;
+    return acc;
+  } else {
+    // gradient2d.c:18-22: C + 1.0f / sqrt(eps + (C-N)^2 + (C-S)^2 + (C-E)^2 + (C-W)^2)
+    // with C's promotion: the float sum goes through the double sqrt.
+    const T cc = get.template at<0, 0, 0>();
+    const T dn = cc - get.template at<-1, 0, 0>();
+    const T ds = cc - get.template at<1, 0, 0>();
+    const T de = cc - get.template at<0, 0, 1>();
+    const T dw = cc - get.template at<0, 0, -1>();
+    T sum = c[0] + dn * dn;
+    sum = sum + ds * ds;
+    sum = sum + de * de;
+    sum = sum + dw * dw;
+    double q = 1.0 / __builtin_sqrt((double)sum);
+    if (gscale != 1.0) q = gscale * q;
+    return (T)((double)cc + q);
+  }
+}
+
+template <typename T, int CJ, int CK>
+struct Plane { T v[CJ][CK]; };
+
+// Stage input of output cell (JJ, KK) of a lane: taps on extended planes read
+// the extension x (rows -XJ.., cells -XK..), the rest read the ring directly.
+template <class D, typename T, int NXA, int CJ, int CK, int XJ, int XK, int JJ, int KK>
+struct RegGet {
+  const T (&x)[NXA][CJ + 2 * XJ][CK + 2 * XK];
+  const Plane<T, CJ, CK> (&rin)[2 * D::rad + 1];
+  template <int A0, int A1, int A2>
+  __device__ __forceinline__ T at() const {
+    constexpr int SL = ext_slot<D>(A0);
+    if constexpr (SL >= 0) return x[SL][JJ + A1 + XJ][KK + A2 + XK];
+    else return rin[D::rad + A0].v[JJ][KK];
+  }
+};
+
+// Stage input straight from global memory (direct kernel).
+template <typename T>
+struct MemGet {
+  const T* p;
+  long long s0, s1;
+  template <int A0, int A1, int A2>
+  __device__ __forceinline__ T at() const { return p[A0 * s0 + A1 * s1 + A2]; }
+};
+
+template <int R>
+constexpr int default_unroll() { return R <= 2 ? 2 * R + 1 : 1; }
+
+// Rows a wave publishes per extended plane for the middle-dim exchange: its
+// first and last XJ rows when it holds at least 2*XJ rows (neighbours need
+// nothing else), otherwise all of them.
+template <int CJ, int XJ>
+constexpr int edge_rows() { return CJ >= 2 * XJ ? 2 * XJ : CJ; }
+// storage index of own row RR among the published rows (-1: not published)
+template <int CJ, int XJ>
+constexpr int edge_index(int rr) {
+  if (CJ < 2 * XJ) return rr;
+  if (rr < XJ) return rr;
+  if (rr >= CJ - XJ) return rr - (CJ - 2 * XJ);
+  return -1;
+}
+
+// LDS bytes of the middle-dim exchange (two buffers).
+template <int ID, typename T, int CK, int CJ, int NW>
+constexpr size_t sweep_lds_bytes() {
+  using D = Def<ID>;
+  if constexpr (D::dim == 3 && needs_mid<D>())
+    return 2ull * num_ext_planes<D>() * NW * edge_rows<CJ, D::rad>() * 64 * CK * sizeof(T);
+  else
+    return 0;
+}
+
+template <int ID, typename T, int BT, int CK, int CJ, int NW, int UNR = default_unroll<Def<ID>::rad>()>
+__global__ __launch_bounds__(64 * NW) void sweep_kernel(SweepArgs<T> a) {
+  using D = Def<ID>;
+  constexpr int R = D::rad;
+  constexpr int P = 2 * R + 1;
+  constexpr bool DIM3 = D::dim == 3;
+  constexpr int NT = D::linear ? D::ntaps : 1;
+  constexpr int NX = num_ext_planes<D>();
+  constexpr bool MID = DIM3 && needs_mid<D>();
+  constexpr bool FAST = needs_fast<D>();
+  constexpr bool CORNER = needs_corners<D>();
+  constexpr int XJ = MID ? R : 0;
+  constexpr int XK = FAST ? R : 0;
+  constexpr int H = BT * R;
+  static_assert(!FAST || CK >= R, "a lane must hold at least R cells");
+  static_assert(DIM3 || CJ == 1, "2D sweeps hold one row per lane");
+  static_assert(!MID || NW * CJ > 2 * H, "tile too thin for BT");
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  int tk, tj = 0, seg;
+  if constexpr (DIM3) {
+    int b = blockIdx.x;
+    tk = b % a.ntk;
+    b /= a.ntk;
+    tj = b % a.ntj;
+    seg = b / a.ntj;
+  } else {
+    const int gw = blockIdx.x * NW + w;
+    if (gw >= a.ntk * a.nseg) return;
+    tk = gw % a.ntk;
+    seg = gw / a.ntk;
+  }
+
+  const int kv0 = R + tk * a.SK;
+  const int kv1 = min(kv0 + a.SK, a.D2 - R);
+  const int kl = kv0 - H + lane * CK;
+  int jw = 0, jv0 = 0, jv1 = 1;
+  if constexpr (DIM3) {
+    jv0 = R + tj * a.SJ;
+    jv1 = min(jv0 + a.SJ, a.D1 - R);
+    jw = jv0 - H + w * CJ;
+  }
+  const int i0 = R + seg * a.sl;
+  const int sle = min(a.sl, a.D0 - R - i0);
+  const int nit = sle + 2 * H;
+
+  bool kin[CK], kst[CK], jin[CJ], jst[CJ];
+  bool kst_all = true;
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) {
+    const int k = kl + kk;
+    kin[kk] = k >= R && k < a.D2 - R;
+    kst[kk] = k >= kv0 && k < kv1;
+    kst_all = kst_all && kst[kk];
+  }
+#pragma unroll
+  for (int jj = 0; jj < CJ; jj++) {
+    const int j = jw + jj;
+    jin[jj] = !DIM3 || (j >= R && j < a.D1 - R);
+    jst[jj] = !DIM3 || (j >= jv0 && j < jv1);
+  }
+  const bool kfull = kl >= 0 && kl + CK <= a.D2;
+
+  T c[NT + 1];
+#pragma unroll
+  for (int q = 0; q < NT + 1; q++) c[q] = a.coef[q];
+  const T div = c[NT];
+  const bool has_div = a.has_div != 0;
+  const double gscale = a.gscale;
+
+  extern __shared__ __align__(16) unsigned char an5d_lds[];
+  int xbuf = 0;
+
+  Plane<T, CJ, CK> ring[BT][P];
+
+  auto load_plane = [&](Plane<T, CJ, CK>& pl, int p) {
+    const T* base = a.src + (long long)p * a.s0;
+#pragma unroll
+    for (int jj = 0; jj < CJ; jj++) {
+      const int j = jw + jj;
+      if (DIM3 && (j < 0 || j >= a.D1)) continue;
+      const T* row = base + (DIM3 ? (long long)j * a.s1 : 0ll) + kl;
+      if (kfull) {
+        load_run<T, CK>(pl.v[jj], row);
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++)
+          if (kl + kk >= 0 && kl + kk < a.D2) pl.v[jj][kk] = row[kk];
+      }
+    }
+  };
+
+  auto store_plane = [&](const Plane<T, CJ, CK>& pl, int q) {
+    T* base = a.dst + (long long)q * a.s0;
+#pragma unroll
+    for (int jj = 0; jj < CJ; jj++) {
+      if (!jst[jj]) continue;
+      T* row = base + (DIM3 ? (long long)(jw + jj) * a.s1 : 0ll) + kl;
+      if (kst_all) {
+        store_run<T, CK>(row, pl.v[jj]);
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++)
+          if (kst[kk]) row[kk] = pl.v[jj][kk];
+      }
+    }
+  };
+
+  // Stage S output plane from ring[S-1]; q = global index of that plane.
+  auto compute = [&](const Plane<T, CJ, CK>(&rin)[P], Plane<T, CJ, CK>& out, int q) {
+    // extended copies of the planes that need in-plane neighbours
+    T x[NX > 0 ? NX : 1][CJ + 2 * XJ][CK + 2 * XK];
+    static_for<-R, R + 1>([&](auto dc) {
+      constexpr int DD = decltype(dc)::value;
+      constexpr int SL = ext_slot<D>(DD);
+      if constexpr (SL >= 0) {
+        const auto& src = rin[R + DD];
+#pragma unroll
+        for (int jj = 0; jj < CJ; jj++) {
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) x[SL][jj + XJ][kk + XK] = src.v[jj][kk];
+          if constexpr (XK > 0) {
+#pragma unroll
+            for (int m = 1; m <= XK; m++) {
+              x[SL][jj + XJ][XK - m] = from_left(src.v[jj][CK - m]);
+              x[SL][jj + XJ][XK + CK - 1 + m] = from_right(src.v[jj][m - 1]);
+            }
+          }
+        }
+      }
+    });
+    if constexpr (MID) {
+      // publish own edge rows, then fetch the XJ rows above and below
+      constexpr int ER = edge_rows<CJ, XJ>();
+      T* lds = reinterpret_cast<T*>(an5d_lds) + (size_t)xbuf * NX * NW * ER * 64 * CK;
+      static_for<0, NX>([&](auto sc) {
+        constexpr int SL = decltype(sc)::value;
+        static_for<0, CJ>([&](auto jc) {
+          constexpr int JJ = decltype(jc)::value;
+          constexpr int E = edge_index<CJ, XJ>(JJ);
+          if constexpr (E >= 0) {
+            T run[CK];
+#pragma unroll
+            for (int kk = 0; kk < CK; kk++) run[kk] = x[SL][JJ + XJ][kk + XK];
+            store_run<T, CK>(lds + ((((size_t)SL * NW + w) * ER + E) * 64 + lane) * CK, run);
+          }
+        });
+      });
+      __syncthreads();
+      static_for<0, NX>([&](auto sc) {
+        constexpr int SL = decltype(sc)::value;
+        static_for<0, 2 * XJ>([&](auto hc) {
+          constexpr int HI = decltype(hc)::value;
+          constexpr int JR = HI < XJ ? HI - XJ : CJ + (HI - XJ);  // row relative to own first row
+          constexpr int WOFF = JR >= 0 ? JR / CJ : -((-JR + CJ - 1) / CJ);
+          constexpr int RR = JR - WOFF * CJ;
+          constexpr int E = edge_index<CJ, XJ>(RR);
+          static_assert(E >= 0, "halo row must be published");
+          int ws = w + WOFF;
+          ws = ws < 0 ? 0 : (ws >= NW ? NW - 1 : ws);
+          T run[CK];
+          load_run<T, CK>(run, lds + ((((size_t)SL * NW + ws) * ER + E) * 64 + lane) * CK);
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) x[SL][JR + XJ][kk + XK] = run[kk];
+          if constexpr (CORNER && XK > 0) {
+#pragma unroll
+            for (int m = 1; m <= XK; m++) {
+              x[SL][JR + XJ][XK - m] = from_left(run[CK - m]);
+              x[SL][JR + XJ][XK + CK - 1 + m] = from_right(run[m - 1]);
+            }
+          }
+        });
+      });
+      xbuf ^= 1;
+    }
+    const bool pin = q >= R && q < a.D0 - R;
+    static_for<0, CJ>([&](auto jc) {
+      constexpr int JJ = decltype(jc)::value;
+      static_for<0, CK>([&](auto kc) {
+        constexpr int KK = decltype(kc)::value;
+        RegGet<D, T, (NX > 0 ? NX : 1), CJ, CK, XJ, XK, JJ, KK> get{x, rin};
+        const T val = eval_point<D, T>(get, c, div, has_div, gscale);
+        const T ctr = rin[R].v[JJ][KK];
+        out.v[JJ][KK] = (pin && jin[JJ] && kin[KK]) ? val : ctr;
+      });
+    });
+  };
+
+  for (int n0 = 0; n0 < nit; n0 += UNR) {
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int n = n0 + u;
+      if (n < nit) {
+        const int p_in = i0 - H + n;
+#pragma unroll
+        for (int p = 0; p < P - 1; p++) ring[0][p] = ring[0][p + 1];
+        if (p_in >= 0 && p_in < a.D0) load_plane(ring[0][P - 1], p_in);
+        static_for<1, BT + 1>([&](auto sc) {
+          constexpr int S = decltype(sc)::value;
+          Plane<T, CJ, CK> out;
+          if (n >= 2 * S * R) compute(ring[S - 1], out, p_in - S * R);
+          if constexpr (S < BT) {
+#pragma unroll
+            for (int p = 0; p < P - 1; p++) ring[S][p] = ring[S][p + 1];
+            ring[S][P - 1] = out;
+          } else {
+            if (n >= 2 * BT * R) store_plane(out, p_in - BT * R);
+          }
+        });
+      }
+    }
+  }
+}
+
+// One step without temporal blocking: every thread evaluates CK adjacent
+// interior cells straight from global memory.  Used where the register
+// pipeline does not fit (the radius-2..4 3D boxes) and as a baseline.
+template <int ID, typename T, int CK>
+__global__ __launch_bounds__(256) void direct_kernel(SweepArgs<T> a) {
+  using D = Def<ID>;
+  constexpr int R = D::rad;
+  constexpr bool DIM3 = D::dim == 3;
+  constexpr int NT = D::linear ? D::ntaps : 1;
+  const int nk = a.D2 - 2 * R;
+  const int nkt = (nk + CK - 1) / CK;
+  const int nj = DIM3 ? a.D1 - 2 * R : 1;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nkt * nj * (a.D0 - 2 * R);
+  if (gid >= total) return;
+  const int kt = (int)(gid % nkt);
+  const long long rest = gid / nkt;
+  const int j = DIM3 ? (int)(rest % nj) + R : 0;
+  const int i = (int)(DIM3 ? rest / nj : rest) + R;
+  const long long base = (long long)i * a.s0 + (DIM3 ? (long long)j * a.s1 : 0ll);
+  T c[NT + 1];
+#pragma unroll
+  for (int q = 0; q < NT + 1; q++) c[q] = a.coef[q];
+  const bool has_div = a.has_div != 0;
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) {
+    const int k = R + kt * CK + kk;
+    if (k >= a.D2 - R) break;
+    const T* p = a.src + base + k;
+    MemGet<T> get{p, a.s0, a.s1};
+    a.dst[base + k] = eval_point<D, T>(get, c, c[NT], has_div, a.gscale);
+  }
+}
+
+}  // namespace an5d

@@ -1,0 +1,253 @@
+"""Python binding of liban5d (the C ABI in include/an5d.h) via ctypes.
+
+Device-resident calls take torch tensors (ROCm build) for memory and the
+current stream; all compute runs in the HIP kernels of liban5d.so.  There is
+no CPU fallback: if the library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liban5d.so")
+
+OK = 0
+FLOAT32, FLOAT64 = 0, 1
+
+
+class An5dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"an5d error {code}: {msg}")
+        self.code = code
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("stencil", ctypes.c_int), ("dtype", ctypes.c_int),
+                ("n", ctypes.c_longlong * 3), ("timesteps", ctypes.c_int),
+                ("coefs", ctypes.POINTER(ctypes.c_double)), ("divisor", ctypes.c_double)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("bt", ctypes.c_int), ("variant", ctypes.c_int), ("sl", ctypes.c_int)]
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("s0", ctypes.c_longlong), ("s1", ctypes.c_longlong), ("buf_stride", ctypes.c_longlong)]
+
+
+class StencilInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("dim", ctypes.c_int), ("rad", ctypes.c_int),
+                ("fpp", ctypes.c_int), ("ntaps", ctypes.c_int), ("divisor", ctypes.c_int),
+                ("linear", ctypes.c_int)]
+
+
+class VariantInfo(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("ck", ctypes.c_int), ("cj", ctypes.c_int),
+                ("nw", ctypes.c_int), ("btmax", ctypes.c_int), ("lds_bytes", ctypes.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C an5d-artifact_amd` "
+                               "(there is no fallback path)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.an5d_last_error.restype = ctypes.c_char_p
+        L.an5d_stencil_id.argtypes = [ctypes.c_char_p]
+        L.an5d_stencil_get_info.argtypes = [ctypes.c_int, P(StencilInfo)]
+        L.an5d_stencil_taps.argtypes = [ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double), ctypes.c_int]
+        L.an5d_variant_count.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.an5d_variant_get_info.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(VariantInfo)]
+        L.an5d_schedule.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_int), ctypes.c_int]
+        L.an5d_kernel_stencil.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
+        L.an5d_forward_host.argtypes = [P(Problem), ctypes.c_void_p, P(Config), P(ctypes.c_double)]
+        L.an5d_forward_device.argtypes = [P(Problem), ctypes.c_void_p, P(Layout), P(Config), ctypes.c_void_p]
+        L.an5d_launch.argtypes = [P(Problem), ctypes.c_void_p, P(Layout), P(Config), ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_void_p]
+        L.an5d_init_grid.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                     ctypes.c_int]
+        L.an5d_cpu_forward.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise An5dError(rc, lib().an5d_last_error().decode())
+    return rc
+
+
+def stencil_id(name: str) -> int:
+    return _check(lib().an5d_stencil_id(name.encode()))
+
+
+def info(name):
+    st = StencilInfo()
+    _check(lib().an5d_stencil_get_info(stencil_id(name), ctypes.byref(st)))
+    return dict(name=st.name.decode(), dim=st.dim, rad=st.rad, fpp=st.fpp, ntaps=st.ntaps,
+                divisor=st.divisor, linear=bool(st.linear))
+
+
+def stencil_names():
+    out = []
+    for i in range(lib().an5d_stencil_count()):
+        st = StencilInfo()
+        _check(lib().an5d_stencil_get_info(i, ctypes.byref(st)))
+        out.append(st.name.decode())
+    return out
+
+
+def taps(name):
+    n = info(name)["ntaps"]
+    offs = (ctypes.c_int * (4 * n))()
+    coefs = (ctypes.c_double * n)()
+    _check(lib().an5d_stencil_taps(stencil_id(name), offs, coefs, n))
+    return np.array(list(offs), dtype=np.int64).reshape(n, 4), np.array(list(coefs))
+
+
+def variants(name, dtype=FLOAT32):
+    sid = stencil_id(name)
+    out = []
+    for v in range(_check(lib().an5d_variant_count(sid, dtype))):
+        vi = VariantInfo()
+        _check(lib().an5d_variant_get_info(sid, dtype, v, ctypes.byref(vi)))
+        out.append(dict(index=v, kind=vi.kind, ck=vi.ck, cj=vi.cj, nw=vi.nw, btmax=vi.btmax,
+                        lds_bytes=vi.lds_bytes))
+    return out
+
+
+def schedule(T, bt):
+    n = _check(lib().an5d_schedule(int(T), int(bt), None, 0))
+    buf = (ctypes.c_int * max(n, 1))()
+    _check(lib().an5d_schedule(int(T), int(bt), buf, n))
+    return list(buf)[:n]
+
+
+def _dtype_code(dt):
+    dt = np.dtype(dt)
+    if dt == np.float32:
+        return FLOAT32
+    if dt == np.float64:
+        return FLOAT64
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+def make_problem(name, n, T, dtype, coefs=None, divisor=-1.0):
+    p = Problem()
+    p.stencil = stencil_id(name)
+    p.dtype = _dtype_code(dtype)
+    for d, v in enumerate(n):
+        p.n[d] = int(v)
+    p.timesteps = int(T)
+    keep = None
+    if coefs is not None:
+        keep = np.ascontiguousarray(np.asarray(coefs, dtype=np.float64))
+        p.coefs = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    p.divisor = float(divisor)
+    return p, keep
+
+
+def make_config(bt=0, variant=-1, sl=0):
+    c = Config()
+    c.bt, c.variant, c.sl = int(bt), int(variant), int(sl)
+    return c
+
+
+def forward_host(name, A, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0):
+    """GPU forward on a host numpy array A[2, ...] (reference layout), in place; returns ms."""
+    assert A.flags.c_contiguous
+    p, keep = make_problem(name, n, T, A.dtype, coefs, divisor)
+    cfg = make_config(bt, variant, sl)
+    ms = ctypes.c_double()
+    _check(lib().an5d_forward_host(ctypes.byref(p), A.ctypes.data, ctypes.byref(cfg), ctypes.byref(ms)))
+    del keep
+    return ms.value
+
+
+def cpu_forward(name, A, n, T, coefs=None, divisor=-1.0, threads=0):
+    """The library's CPU path (OpenMP loop) on a host array, in place; returns ms."""
+    assert A.flags.c_contiguous
+    p, keep = make_problem(name, n, T, A.dtype, coefs, divisor)
+    ms = ctypes.c_double()
+    _check(lib().an5d_cpu_forward(ctypes.byref(p), A.ctypes.data, int(threads), ctypes.byref(ms)))
+    del keep
+    return ms.value
+
+
+def kernel_stencil(name, A, compsize, T, scop=True):
+    """The drop-in entry (common.h:42 signature) on a host array; returns ms."""
+    ms = ctypes.c_double()
+    _check(lib().an5d_kernel_stencil(stencil_id(name), _dtype_code(A.dtype), A.ctypes.data,
+                                     int(compsize), int(T), 1 if scop else 0, ctypes.byref(ms)))
+    return ms.value
+
+
+def init_grid(name, n, dtype=np.float32, mode=0, seed_base=None):
+    """Both buffers of a host grid for interior extents n: mode 0 is the
+    reference's init_grid (seed i + compsize), mode 1 uniform [0, 1)."""
+    d = info(name)
+    shape = (2,) + tuple(int(x) + 2 * d["rad"] for x in n)
+    A = np.empty(shape, dtype=dtype)
+    if seed_base is None:
+        seed_base = int(n[0])
+    _check(lib().an5d_init_grid(_dtype_code(dtype), A.ctypes.data, A[0].size, int(seed_base), int(mode)))
+    return A
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def layout_of(t, dim):
+    """Strides (elements) of a torch tensor shaped [2, D0, (D1,) D2] with unit last stride."""
+    st = t.stride()
+    if st[-1] != 1:
+        raise ValueError("last dimension must be contiguous")
+    L = Layout()
+    L.buf_stride = st[0]
+    L.s0 = st[1]
+    L.s1 = st[2] if dim == 3 else st[1]
+    return L
+
+
+def forward_device(name, t, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, stream=None):
+    """Enqueue T steps on a device tensor t[2, ...]; returns immediately."""
+    d = info(name)
+    p, keep = make_problem(name, n, T, _np_dtype(t), coefs, divisor)
+    cfg = make_config(bt, variant, sl)
+    L = layout_of(t, d["dim"])
+    _check(lib().an5d_forward_device(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()), ctypes.byref(L),
+                                     ctypes.byref(cfg), _stream_ptr(stream)))
+    del keep
+
+
+def launch(name, t, n, steps, c0=0, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, stream=None):
+    """One launch of `steps` fused steps from buffer c0%2 to (c0+1)%2."""
+    d = info(name)
+    p, keep = make_problem(name, n, steps, _np_dtype(t), coefs, divisor)
+    cfg = make_config(bt if bt else steps, variant, sl)
+    L = layout_of(t, d["dim"])
+    _check(lib().an5d_launch(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()), ctypes.byref(L),
+                             ctypes.byref(cfg), int(steps), int(c0), _stream_ptr(stream)))
+    del keep
+
+
+def _np_dtype(t):
+    import torch
+    if t.dtype == torch.float32:
+        return np.float32
+    if t.dtype == torch.float64:
+        return np.float64
+    raise ValueError(f"unsupported tensor dtype {t.dtype}")

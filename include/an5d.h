@@ -1,0 +1,150 @@
+/*
+ * an5d.h -- C ABI of the MI355X temporal-blocking stencil engine (liban5d.so).
+ *
+ * Plain C types only: pointers, sizes, ints and doubles.  Every entry point
+ * returns an int status (AN5D_OK == 0); on failure an5d_last_error() holds a
+ * message for the calling thread.  The library never aborts and never
+ * exits; the benchmark harness maps failures onto the reference's
+ * stderr-and-abort behaviour.
+ *
+ * Data layout (the reference's, common.h:44-56 and j2d5pt.c:10-11): a
+ * problem with interior extents n[] and radius r owns two buffers of
+ * D = n + 2r cells per dimension, stored back to back ("buffer-major"),
+ * each row-major with the last dimension fastest.  The halo (r cells on
+ * every side) holds fixed (Dirichlet) values; it is read, never written.
+ * After T steps buffer T%2 holds u_T.
+ */
+#ifndef AN5D_H
+#define AN5D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AN5D_OK 0
+#define AN5D_E_INVALID (-1)      /* bad argument */
+#define AN5D_E_UNSUPPORTED (-2)  /* no compiled kernel fits the request */
+#define AN5D_E_HIP (-3)          /* HIP runtime error */
+#define AN5D_E_ALLOC (-4)        /* host or device allocation failed */
+
+#define AN5D_FLOAT32 0
+#define AN5D_FLOAT64 1
+
+/* A stencil problem.  `stencil` is an id from an5d_stencil_id().  n[] are
+ * interior extents, outermost first (2D uses n[0], n[1]; the reference only
+ * has squares, n[d] == compsize).  coefs == NULL selects the source's float
+ * literals (FP64 uses (double)(float)literal, as the reference's
+ * `5.1f * A[..]` does in double code, j2d5pt.c:18); otherwise it holds one
+ * value per tap in source term order.  For gradient2d coefs is
+ * {eps, scale}: out = C + scale / sqrt(eps + sum (C-N)^2) (reference:
+ * eps = 0.0001f, scale = 1).  divisor < 0 keeps the source's divisor
+ * (118 or 159 or none), 0 removes it, > 0 replaces it. */
+typedef struct an5d_problem {
+  int stencil;
+  int dtype;
+  long long n[3];
+  int timesteps;
+  const double* coefs;
+  double divisor;
+} an5d_problem;
+
+/* Tiling.  bt: timesteps fused per launch (0: tuned/default).  variant:
+ * index into an5d_variant_info() for this stencil x dtype (-1: tuned or
+ * default).  sl: planes per streaming segment along dim 0 (0: default). */
+typedef struct an5d_config {
+  int bt;
+  int variant;
+  int sl;
+} an5d_config;
+
+/* Element strides of a device-resident problem: cell (i,j,k) of buffer b is
+ * at base[b*buf_stride + i*s0 + j*s1 + k] (2D: i*s0 + k).  The reference
+ * layout is s1 = D2, s0 = D1*D2 (2D: s0 = D1), buf_stride = D0*s0. */
+typedef struct an5d_layout {
+  long long s0, s1, buf_stride;
+} an5d_layout;
+
+typedef struct an5d_stencil_info {
+  char name[32];
+  int dim, rad, fpp, ntaps, divisor, linear;
+} an5d_stencil_info;
+
+typedef struct an5d_variant_info {
+  int kind;  /* 0: temporally blocked sweep, 1: one step per launch */
+  int ck, cj, nw, btmax;
+  long long lds_bytes;
+} an5d_variant_info;
+
+/* ---- registry ------------------------------------------------------------
+ * Replaces the compile-time BENCH_DIM/BENCH_FPP/BENCH_RAD macros of each
+ * <stencil>.c (j2d5pt.c:1-3) with a runtime table of all 21 stencils. */
+int an5d_stencil_count(void);
+int an5d_stencil_id(const char* name); /* "j2d5pt", "j2d9pt-gol", ... ; <0 if unknown */
+int an5d_stencil_get_info(int stencil, an5d_stencil_info* out);
+/* taps: 4 ints per tap (sign, d_outer, d_mid, d_fast) and the literal coefficients */
+int an5d_stencil_taps(int stencil, int* offsets, double* coefs, int max_taps);
+int an5d_variant_count(int stencil, int dtype);
+int an5d_variant_get_info(int stencil, int dtype, int variant, an5d_variant_info* out);
+
+/* ---- launch schedule ----------------------------------------------------
+ * The step counts of the launches that advance T steps at temporal block bt
+ * so that u_T lands in buffer T%2: floor(T/bt) blocks of bt, with the tail
+ * re-split when the launch count's parity differs from T's.  Same rule as
+ * the generated host launcher (compiled/float/j2d5pt-256-10-128_host.cu:72-554).
+ * Returns the number of launches written to steps[] (or <0). */
+int an5d_schedule(int timesteps, int bt, int* steps, int max_launches);
+
+/* ---- forward sweep --------------------------------------------------------
+ * Drop-in core of `double kernel_stencil(SB_TYPE *A1, int compsize, int
+ * timestep, bool scop)` (common.h:42; generated host.cu:10).  A1 is the
+ * caller's host array (2 buffers, reference layout, square grid).  scop != 0
+ * runs on the GPU: device buffers (cached across calls), H2D copy, sync,
+ * timed launch schedule, sync, D2H copy (host.cu:34-41,74-77,556-564);
+ * scop == 0 runs the CPU path (the OpenMP loop, host.cu:567-575).  *ms gets
+ * the elapsed milliseconds of the timed region (kernels only on the GPU,
+ * the whole loop on the CPU, as in the reference).  Does nothing when
+ * D < 2r+1 or timestep < 1 (host.cu:17). */
+int an5d_kernel_stencil(int stencil, int dtype, void* A1, int compsize, int timestep,
+                        int scop, double* ms);
+
+/* Same as the GPU branch above for any problem (rectangular grids, other
+ * coefficients) and tiling.  A: host array, reference layout. */
+int an5d_forward_host(const an5d_problem* prob, void* A, const an5d_config* cfg, double* ms);
+
+/* Device-resident form: dA holds both buffers in `layout` (NULL: reference
+ * layout).  Enqueues the launch schedule on `stream` (a hipStream_t, NULL:
+ * the null stream) and returns without synchronising.  c0 is the launch
+ * ordinal of the first launch (0: read buffer 0).  No replacement in the
+ * reference, whose kernels are only reachable through kernel_stencil. */
+int an5d_forward_device(const an5d_problem* prob, void* dA, const an5d_layout* layout,
+                        const an5d_config* cfg, void* stream);
+
+/* One launch of `steps` fused steps reading buffer c0%2 and writing
+ * (c0+1)%2: the kernel0_<steps>(A, dimsize, timestep, c0) launch of the
+ * generated host (compiled/float/j2d5pt-256-10-128_kernel.hu:3-12). */
+int an5d_launch(const an5d_problem* prob, void* dA, const an5d_layout* layout,
+                const an5d_config* cfg, int steps, int c0, void* stream);
+
+/* The CPU path: T steps of the OpenMP loop (j2d5pt.c:23-31) on a host
+ * array in the reference layout; threads <= 0 uses the OpenMP default. */
+int an5d_cpu_forward(const an5d_problem* prob, void* A, int threads, double* ms);
+
+/* Fill both buffers of a host grid (cells = cells per buffer, halo
+ * included) from rand_r seeded with index + seed_base, in parallel.
+ * mode 0 is the reference's init_grid (common.h:44-56, seed_base =
+ * compsize): 1000*(T)rand_r/(T)RAND_MAX.  mode 1: rand_r/(RAND_MAX+1.0),
+ * uniform in [0, 1). */
+int an5d_init_grid(int dtype, void* A, long long cells, long long seed_base, int mode);
+
+/* Release cached device buffers. */
+int an5d_release(void);
+
+const char* an5d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AN5D_H */

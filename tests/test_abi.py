@@ -1,0 +1,93 @@
+"""The C ABI library loads, exports every symbol include/*.h declares, and
+its registry and argument checking work without touching a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pyan5d
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("an5d.h",):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(an5d_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = pyan5d.lib()
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_kernel_stencil_macro_header_compiles(tmp_path):
+    src = tmp_path / "ks.c"
+    src.write_text('#define SB_TYPE float\n#include "an5d_kernel_stencil.h"\n'
+                   'AN5D_DEFINE_KERNEL_STENCIL("j2d5pt")\n')
+    rc = os.system(f"gcc -c -I{ROOT}/include {src} -o {tmp_path}/ks.o")
+    assert rc == 0
+
+
+def test_registry_matches_oracle():
+    names = pyan5d.stencil_names()
+    assert names == O.NAMES
+    for name in names:
+        a, b = pyan5d.info(name), O.info(name)
+        for k in ("dim", "rad", "fpp", "ntaps", "divisor", "linear"):
+            assert a[k] == b[k], (name, k)
+        if a["linear"]:
+            po, pc = pyan5d.taps(name)
+            oo, oc = O.taps(name)
+            assert np.array_equal(po, oo)
+            assert np.array_equal(pc.astype(np.float32), oc)
+
+
+@pytest.mark.parametrize("name", O.NAMES)
+def test_every_stencil_has_kernels_for_both_dtypes(name):
+    for dt in (0, 1):
+        vs = pyan5d.variants(name, dt)
+        assert vs, name
+        assert vs[-1]["kind"] == 1  # the one-step-per-launch kernel is always there
+        for v in vs:
+            assert 1 <= v["btmax"] <= 16
+
+
+def test_argument_errors_do_not_touch_the_gpu():
+    with pytest.raises(pyan5d.An5dError):
+        pyan5d.stencil_id("nope")
+    A = np.zeros((2, 6, 6), np.float32)
+    with pytest.raises(pyan5d.An5dError) as e:
+        pyan5d.forward_host("j2d5pt", A, [0, 4], 3)
+    assert e.value.code == -1
+    with pytest.raises(pyan5d.An5dError):
+        pyan5d.forward_host("j2d5pt", A, [4, 4], -1)
+    p, _ = pyan5d.make_problem("j2d5pt", [4, 4], 3, np.float32)
+    p.dtype = 7
+    ms = ctypes.c_double()
+    assert pyan5d.lib().an5d_forward_host(ctypes.byref(p), A.ctypes.data, None, ctypes.byref(ms)) == -1
+    p.dtype = 0
+    assert pyan5d.lib().an5d_forward_host(ctypes.byref(p), None, None, ctypes.byref(ms)) == -1
+    cfg = pyan5d.make_config(bt=0, variant=99)
+    # variant out of range is rejected before any device work
+    assert pyan5d.lib().an5d_forward_device(ctypes.byref(p), ctypes.c_void_p(16), None,
+                                            ctypes.byref(cfg), None) == -1
+
+
+def test_zero_steps_is_a_no_op():
+    A = np.arange(2 * 36, dtype=np.float32).reshape(2, 6, 6)
+    B = A.copy()
+    pyan5d.forward_host("j2d5pt", A, [4, 4], 0)
+    assert np.array_equal(A, B)
+    assert pyan5d.kernel_stencil("j2d5pt", A, 4, 0, scop=True) == 0.0
+    assert np.array_equal(A, B)

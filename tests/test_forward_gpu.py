@@ -1,0 +1,176 @@
+"""GPU parity of the forward sweep: the HIP kernels against the CPU oracle,
+bit for bit (every stencil x dtype x kernel shape x several bt), tiling
+invariance, the kernel_stencil drop-in, device-resident layouts and the
+golden fixtures made from the reference's own programs."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pyan5d
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def small_n(name):
+    d = O.info(name)
+    if d["dim"] == 2:
+        return [37, 300]  # two column tiles, ragged edges
+    if d["ntaps"] > 100:
+        return [6, 7, 9]
+    return [11, 45, 150]
+
+
+def rand_grid(name, n, dt, seed):
+    d = O.info(name)
+    rng = np.random.default_rng(seed)
+    shape = tuple(x + 2 * d["rad"] for x in n)
+    u = rng.random(shape).astype(dt)
+    return np.stack([u, u])
+
+
+def same(a, b):
+    return np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def cases():
+    out = []
+    for name in O.NAMES:
+        for dt in (np.float32, np.float64):
+            for v in pyan5d.variants(name, 0 if dt == np.float32 else 1):
+                bts = sorted({1, v["btmax"], max(1, v["btmax"] // 2)})
+                for bt in bts:
+                    out.append((name, dt, v["index"], bt))
+    return out
+
+
+@pytest.mark.parametrize("name,dt,variant,bt", cases())
+def test_forward_bitwise_vs_oracle(name, dt, variant, bt):
+    n = small_n(name)
+    T = 2 * bt + 1 if O.info(name)["ntaps"] <= 100 else 2
+    print(f"case {name} {np.dtype(dt).name} variant={variant} bt={bt} n={n} T={T}", flush=True)
+    A = rand_grid(name, n, dt, seed=bt * 7 + variant)
+    B = A.copy()
+    pyan5d.forward_host(name, A, n, T, bt=bt, variant=variant)
+    O.run(name, B, n, T)
+    assert same(A[T % 2], B[T % 2])
+
+
+@pytest.mark.parametrize("name,dt", [("j2d5pt", np.float32), ("star3d1r", np.float32),
+                                     ("box3d1r", np.float64), ("gradient2d", np.float32)])
+def test_tiling_invariance(name, dt):
+    d = O.info(name)
+    n = [300, 1030] if d["dim"] == 2 else [40, 130, 200]
+    T = 13
+    base = rand_grid(name, n, dt, seed=3)
+    results = []
+    for v in pyan5d.variants(name, 0 if dt == np.float32 else 1):
+        for bt in range(1, v["btmax"] + 1):
+            for sl in (0, 7, 64):
+                A = base.copy()
+                print(f"case {name} variant={v['index']} bt={bt} sl={sl}", flush=True)
+                pyan5d.forward_host(name, A, n, T, bt=bt, variant=v["index"], sl=sl)
+                results.append(A[T % 2].copy())
+    B = base.copy()
+    pyan5d.cpu_forward(name, B, n, T)
+    for r in results:
+        assert same(r, B[T % 2])
+
+
+def test_kernel_stencil_dropin_config0_size():
+    """j2d5pt FP32 4096^2, 100 steps, reference init; GPU path vs
+    the CPU path (OpenMP loop), whole grid, both buffers' halo untouched."""
+    print("case j2d5pt 4096^2 T=100 drop-in", flush=True)
+    A = O.init_grid("j2d5pt", 4096, np.float32)
+    B = A.copy()
+    halo = A[:, 0, :].copy()
+    pyan5d.kernel_stencil("j2d5pt", A, 4096, 100, scop=True)
+    pyan5d.kernel_stencil("j2d5pt", B, 4096, 100, scop=False)
+    assert same(A[0], B[0])
+    assert np.array_equal(A[:, 0, :], halo)
+
+
+@pytest.mark.parametrize("name", O.NAMES)
+@pytest.mark.parametrize("dt,tname", [(np.float32, "float"), (np.float64, "double")])
+def test_golden_fixture_on_gpu(name, dt, tname):
+    g = np.load(os.path.join(GOLD, f"stencil_{name}_{tname}.npz"))
+    n, T = int(g["n"]), int(g["T"])
+    print(f"case golden {name} {tname} n={n} T={T}", flush=True)
+    d = O.info(name)
+    A = O.init_grid(name, n, dt)
+    pyan5d.kernel_stencil(name, A, n, T, scop=True)
+    assert same(A[T % 2], g["out"][T % 2])
+
+
+def test_forward_device_layouts(torch_cuda):
+    torch = torch_cuda
+    name, n, T = "star3d1r", [20, 33, 70], 9
+    d = O.info(name)
+    base = rand_grid(name, n, np.float32, seed=11)
+    B = base.copy()
+    O.run(name, B, n, T)
+    # reference layout
+    t = torch.from_numpy(base.copy()).cuda()
+    print("case forward_device reference layout", flush=True)
+    pyan5d.forward_device(name, t, n, T, bt=3)
+    torch.cuda.synchronize()
+    assert same(t.cpu().numpy()[T % 2], B[T % 2])
+    # padded rows and planes (strided view)
+    D = [x + 2 * d["rad"] for x in n]
+    big = torch.zeros((2, D[0], D[1] + 3, D[2] + 29), dtype=torch.float32, device="cuda")
+    view = big[:, :, :D[1], :D[2]]
+    view.copy_(torch.from_numpy(base))
+    print("case forward_device strided layout", flush=True)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        pyan5d.forward_device(name, view, n, T, bt=2, stream=s)
+    s.synchronize()
+    assert same(view.cpu().numpy()[T % 2], B[T % 2])
+
+
+def test_single_launch_entry(torch_cuda):
+    torch = torch_cuda
+    name, n = "j2d9pt", [50, 400]
+    base = rand_grid(name, n, np.float32, seed=2)
+    B = base.copy()
+    O.run(name, B, n, 5)
+    t = torch.from_numpy(base.copy()).cuda()
+    print("case single launches 3+2", flush=True)
+    pyan5d.launch(name, t, n, 3, c0=0)
+    pyan5d.launch(name, t, n, 2, c0=1)
+    torch.cuda.synchronize()
+    assert same(t.cpu().numpy()[0], B[1])
+
+
+@pytest.mark.parametrize("name", ["j2d5pt", "star3d1r", "box2d4r", "j3d27pt", "gradient2d"])
+def test_other_coefficients_and_divisor(name):
+    d = O.info(name)
+    n = [30, 270] if d["dim"] == 2 else [10, 40, 140]
+    rng = np.random.default_rng(1)
+    if d["linear"]:
+        c = rng.random(d["ntaps"]) + 0.01
+        c = c / c.sum()
+        div = 0.0
+    else:
+        c = [1e-4, 0.01]
+        div = -1.0
+    for dt in (np.float32, np.float64):
+        A = rand_grid(name, n, dt, seed=4)
+        B = A.copy()
+        print(f"case coefs {name} {np.dtype(dt).name}", flush=True)
+        pyan5d.forward_host(name, A, n, 6, coefs=c, divisor=div)
+        O.run(name, B, n, 6, coefs=c, divisor=div)
+        assert same(A[0], B[0])
+
+
+def test_rectangular_and_tiny_grids():
+    for name, n in [("star2d1r", [1, 1]), ("star2d2r", [3, 700]), ("j2d5pt", [900, 5]),
+                    ("star3d1r", [1, 2, 3]), ("star3d1r", [70, 3, 9]), ("box3d1r", [5, 60, 2])]:
+        A = rand_grid(name, n, np.float32, seed=9)
+        B = A.copy()
+        print(f"case rect {name} {n}", flush=True)
+        pyan5d.forward_host(name, A, n, 7)
+        O.run(name, B, n, 7)
+        assert same(A[1], B[1])

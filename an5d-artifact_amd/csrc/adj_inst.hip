@@ -1,0 +1,76 @@
+// Instantiates and registers the adjoint kernels of one 2D stencil x dtype
+// (-DAN5D_SID=<stencil id> -DAN5D_F64=<0|1>).
+#include "adjoint.hpp"
+#include "engine.hpp"
+
+namespace an5d {
+namespace {
+
+using T = std::conditional_t<AN5D_F64, double, float>;
+constexpr int SID = AN5D_SID;
+using D = Def<SID>;
+
+// cells per lane and deepest block: keeps fr/ar rings under ~200 VGPRs
+constexpr int adj_ck() {
+  if (!D::linear) return 2;
+  if (D::rad <= 2) return AN5D_F64 ? 2 : 4;
+  return 4;
+}
+constexpr int adj_btmax() {
+  if (!D::linear) return AN5D_F64 ? 3 : 4;
+  if (D::rad == 1) return D::ntaps > 5 ? 3 : 4;
+  if (D::rad == 2) return 2;
+  return 1;
+}
+
+template <int BT>
+hipError_t launch_adj(const AdjLaunch& l, dim3 grid, dim3 block, hipStream_t st) {
+  AdjArgs<T> a;
+  a.U = static_cast<const T*>(l.U);
+  a.Lin = static_cast<const T*>(l.Lin);
+  a.Lout = static_cast<T*>(l.Lout);
+  a.coef = static_cast<const T*>(l.coef);
+  a.gscale = l.gscale;
+  a.has_div = l.has_div;
+  a.gacc = l.gacc;
+  a.D0 = l.D0; a.D2 = l.D2; a.s0 = l.s0;
+  a.ntk = l.ntk; a.nseg = l.nseg; a.sl = l.sl; a.SK = l.SK;
+  hipLaunchKernelGGL((adjoint_kernel<SID, T, BT, adj_ck(), 4>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed(const void* uT, const void* target, void* lam, double* loss, int D0, int D2,
+                       long long s0, int R, hipStream_t st) {
+  hipLaunchKernelGGL((adjoint_seed_kernel<T>), dim3(1024), dim3(256), 0, st, static_cast<const T*>(uT),
+                     static_cast<const T*>(target), static_cast<T*>(lam), loss, D0, D2, s0, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_halo(const void* src, void* dst, int D0, int D2, long long s0, int R, hipStream_t st) {
+  hipLaunchKernelGGL((halo_copy_kernel<T>), dim3(1024), dim3(256), 0, st, static_cast<const T*>(src),
+                     static_cast<T*>(dst), D0, D2, s0, R);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define AN5D_CAT2(a, b, c) a##b##_##c
+#define AN5D_CAT(a, b, c) AN5D_CAT2(a, b, c)
+void AN5D_CAT(register_adjoint_, AN5D_SID, AN5D_F64)(std::vector<AdjVariant>& out) {
+  AdjVariant v{};
+  v.stencil = SID;
+  v.dtype = AN5D_F64 ? kF64 : kF32;
+  v.ck = adj_ck();
+  v.nw = 4;
+  v.btmax = adj_btmax();
+  v.reach = adj_reach<D>();
+  v.seed = &launch_seed;
+  v.halo = &launch_halo;
+  static_for<1, adj_btmax() + 1>([&](auto bc) {
+    constexpr int BT = decltype(bc)::value;
+    v.fn[BT] = &launch_adj<BT>;
+  });
+  out.push_back(v);
+}
+
+}  // namespace an5d

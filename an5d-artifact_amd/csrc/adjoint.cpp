@@ -1,2 +1,268 @@
-// Adjoint entry points (filled in by the adjoint sweep).
+// Adjoint entry points: checkpointed forward, seed, reverse block sweep.
+// No counterpart in the reference (it computes no gradients).
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/an5d.h"
+#include "engine.hpp"
 #include "internal.hpp"
+
+namespace an5d {
+
+#define AN5D_ADJ_DECL_2(id)                                  \
+  void register_adjoint_##id##_0(std::vector<AdjVariant>&); \
+  void register_adjoint_##id##_1(std::vector<AdjVariant>&);
+#define AN5D_ADJ_DECL_3(id)
+#define AN5D_ADJ_DECL(id, ident, dim, rad, fpp, n, div, lin) AN5D_ADJ_DECL_##dim(id)
+AN5D_STENCILS(AN5D_ADJ_DECL)
+
+std::vector<AdjVariant>& adj_registry() {
+  static std::vector<AdjVariant> v = [] {
+    std::vector<AdjVariant> out;
+#define AN5D_ADJ_CALL_2(id)       \
+  register_adjoint_##id##_0(out); \
+  register_adjoint_##id##_1(out);
+#define AN5D_ADJ_CALL_3(id)
+#define AN5D_ADJ_CALL(id, ident, dim, rad, fpp, n, div, lin) AN5D_ADJ_CALL_##dim(id)
+    AN5D_STENCILS(AN5D_ADJ_CALL)
+    return out;
+  }();
+  return v;
+}
+
+static const AdjVariant* adj_variant(int stencil, int dtype) {
+  for (const AdjVariant& v : adj_registry())
+    if (v.stencil == stencil && v.dtype == dtype) return &v;
+  return nullptr;
+}
+
+static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+struct AdjPlan {
+  const Variant* fwd = nullptr;
+  const AdjVariant* adj = nullptr;
+  int bt = 0, sl = 0;
+  std::vector<int> steps;
+  Geom g;
+  size_t grid_bytes = 0, ws_bytes = 0;
+};
+
+static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& ap) {
+  std::string err;
+  if (!make_geom(p, ap.g, err)) return fail(AN5D_E_INVALID, err);
+  if (ap.g.dim != 2) return fail(AN5D_E_UNSUPPORTED, "the adjoint sweep supports 2D stencils");
+  if (p->timesteps < 1) return fail(AN5D_E_INVALID, "adjoint needs timesteps >= 1");
+  ap.adj = adj_variant(p->stencil, p->dtype);
+  if (!ap.adj) return fail(AN5D_E_UNSUPPORTED, "no adjoint kernels for this stencil/dtype");
+  auto vs = variants_of(p->stencil, p->dtype);
+  int vi = cfg && cfg->variant >= 0 ? cfg->variant : 0;
+  if (vi >= (int)vs.size()) return fail(AN5D_E_INVALID, "variant index out of range");
+  ap.fwd = vs[vi];
+  int bt = cfg && cfg->bt > 0 ? cfg->bt : ap.adj->btmax;
+  if (bt > ap.adj->btmax || bt > ap.fwd->btmax) {
+    if (cfg && cfg->bt > 0) return fail(AN5D_E_UNSUPPORTED, "bt exceeds the adjoint or forward btmax");
+    bt = ap.adj->btmax < ap.fwd->btmax ? ap.adj->btmax : ap.fwd->btmax;
+  }
+  if (ap.fwd->kind == kDirect) bt = 1;
+  ap.bt = bt;
+  int n = an5d_schedule(p->timesteps, bt, nullptr, 0);
+  ap.steps.resize(n);
+  an5d_schedule(p->timesteps, bt, ap.steps.data(), n);
+  ap.sl = cfg ? cfg->sl : 0;
+  const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
+  ap.grid_bytes = ((size_t)ap.g.area * esz + 255) / 256 * 256;
+  ap.ws_bytes = ap.grid_bytes * (ap.steps.size() + 1 + 2);
+  return AN5D_OK;
+}
+
+static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block) {
+  const int R = ap.g.r, W = ap.adj->reach;
+  const long long HA = (2LL * k - 2) * R + W;
+  const long long SK = 64LL * ap.adj->ck - 2 * HA;
+  if (SK <= 0) return fail(AN5D_E_UNSUPPORTED, "adjoint block too deep for the tile");
+  const long long nk = ap.g.D2 - 2 * R, n0 = ap.g.D0 - 2 * R;
+  const long long ntk = cdiv(nk, SK);
+  long long sl = ap.sl;
+  if (sl <= 0) {
+    const long long warm = (4LL * k - 4) * R + 2 * W;
+    sl = cdiv(n0, cdiv(4096, ntk));
+    if (sl < 4 * warm) sl = 4 * warm;
+    if (sl < 16) sl = 16;
+  }
+  if (sl > n0) sl = n0;
+  const long long nseg = cdiv(n0, sl);
+  l.D0 = (int)ap.g.D0;
+  l.D2 = (int)ap.g.D2;
+  l.s0 = ap.g.D2;
+  l.ntk = (int)ntk;
+  l.nseg = (int)nseg;
+  l.sl = (int)sl;
+  l.SK = (int)SK;
+  block = dim3(64 * ap.adj->nw);
+  grid = dim3((unsigned)cdiv(ntk * nseg, ap.adj->nw));
+  return AN5D_OK;
+}
+
+}  // namespace an5d
+
+using namespace an5d;
+
+extern "C" int an5d_adjoint_workspace(const an5d_problem* p, const an5d_config* cfg, size_t* bytes) {
+  AdjPlan ap;
+  int rc = plan_adjoint(p, cfg, ap);
+  if (rc) return rc;
+  if (bytes) *bytes = ap.ws_bytes;
+  return AN5D_OK;
+}
+
+extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const void* dtarget,
+                                   void* dgrad, double* dgcoef, double* dloss, void* ws,
+                                   size_t ws_bytes, const an5d_config* cfg, void* stream_v,
+                                   double* ms_fwd) {
+  AdjPlan ap;
+  int rc = plan_adjoint(p, cfg, ap);
+  if (rc) return rc;
+  if (!du0 || !dtarget || !dgrad || !dgcoef || !dloss || !ws)
+    return fail(AN5D_E_INVALID, "null device pointer");
+  if (ws_bytes < ap.ws_bytes) return fail(AN5D_E_INVALID, "workspace too small");
+  hipStream_t st = static_cast<hipStream_t>(stream_v);
+  const Geom& g = ap.g;
+  const int R = g.r;
+  std::vector<unsigned char> cbytes;
+  int has_div = 0;
+  double gscale = 1.0;
+  std::string err;
+  if (!build_coefs(p, cbytes, has_div, gscale, err)) return fail(AN5D_E_INVALID, err);
+  const void* dcoef = nullptr;
+  rc = device_coefs(cbytes, &dcoef);
+  if (rc) return rc;
+  const int L = (int)ap.steps.size();
+  char* base = static_cast<char*>(ws);
+  auto ckpt = [&](int i) { return base + (size_t)i * ap.grid_bytes; };
+  char* lam[2] = {ckpt(L + 1), ckpt(L + 2)};
+  const int nc = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
+  hipError_t e;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (ms_fwd) {
+    (void)hipEventCreate(&ev0);
+    (void)hipEventCreate(&ev1);
+    (void)hipEventRecord(ev0, st);
+  }
+  // checkpoint 0 = u0; every other slot gets u0's halo (forward launches write interiors only)
+  e = hipMemcpyAsync(ckpt(0), du0, (size_t)g.area * (p->dtype == AN5D_FLOAT64 ? 8 : 4),
+                     hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(u0)");
+  for (int i = 1; i <= L; i++) {
+    e = ap.adj->halo(du0, ckpt(i), (int)g.D0, (int)g.D2, g.D2, R, st);
+    if (e != hipSuccess) return hip_fail(e, "halo copy");
+  }
+  // forward with a checkpoint at every launch boundary
+  for (int i = 0; i < L; i++) {
+    LaunchPlan lp;
+    if (!plan_launch(*ap.fwd, ap.steps[i], g, ap.sl, lp, err)) return fail(AN5D_E_UNSUPPORTED, err);
+    lp.args.src = ckpt(i);
+    lp.args.dst = ckpt(i + 1);
+    lp.args.coef = dcoef;
+    lp.args.gscale = gscale;
+    lp.args.has_div = has_div;
+    lp.args.s0 = g.D2;
+    lp.args.s1 = g.D2;
+    e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
+    if (e != hipSuccess) return hip_fail(e, "forward launch");
+  }
+  if (ms_fwd) (void)hipEventRecord(ev1, st);
+  // seed: lambda_T = u_T - target on the interior, loss
+  e = hipMemsetAsync(dloss, 0, sizeof(double), st);
+  if (e == hipSuccess) e = hipMemsetAsync(dgcoef, 0, sizeof(double) * nc, st);
+  if (e == hipSuccess) e = hipMemsetAsync(lam[1], 0, ap.grid_bytes, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  e = ap.adj->seed(ckpt(L), dtarget, lam[0], dloss, (int)g.D0, (int)g.D2, g.D2, R, st);
+  if (e != hipSuccess) return hip_fail(e, "adjoint seed");
+  // reverse blocks
+  int cur = 0;
+  for (int i = L - 1; i >= 0; i--) {
+    const int k = ap.steps[i];
+    AdjLaunch l{};
+    dim3 grid, block;
+    rc = adj_launch_plan(ap, k, l, grid, block);
+    if (rc) return rc;
+    l.U = ckpt(i);
+    l.Lin = lam[cur];
+    l.Lout = lam[cur ^ 1];
+    l.coef = dcoef;
+    l.gscale = gscale;
+    l.has_div = has_div;
+    l.gacc = dgcoef;
+    if (!ap.adj->fn[k]) return fail(AN5D_E_UNSUPPORTED, "no adjoint kernel for this block size");
+    e = ap.adj->fn[k](l, grid, block, st);
+    if (e != hipSuccess) return hip_fail(e, "adjoint launch");
+    cur ^= 1;
+  }
+  e = hipMemcpyAsync(dgrad, lam[cur], (size_t)g.area * (p->dtype == AN5D_FLOAT64 ? 8 : 4),
+                     hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(grad)");
+  if (ms_fwd) {
+    float f = 0.f;
+    (void)hipEventSynchronize(ev1);
+    (void)hipEventElapsedTime(&f, ev0, ev1);
+    *ms_fwd = f;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+  }
+  return AN5D_OK;
+}
+
+extern "C" int an5d_adjoint_host(const an5d_problem* p, const void* u0, const void* target,
+                                 void* grad, double* gcoef, double* loss, const an5d_config* cfg,
+                                 double* ms_fwd, double* ms_total) {
+  AdjPlan ap;
+  int rc = plan_adjoint(p, cfg, ap);
+  if (rc) return rc;
+  if (!u0 || !target || !grad || !gcoef || !loss) return fail(AN5D_E_INVALID, "null host pointer");
+  const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
+  const size_t gb = (size_t)ap.g.area * esz;
+  const int nc = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
+  void *du0 = nullptr, *dt = nullptr, *dg = nullptr, *ws = nullptr;
+  double* dsc = nullptr;
+  hipError_t e = hipMalloc(&du0, gb);
+  if (e == hipSuccess) e = hipMalloc(&dt, gb);
+  if (e == hipSuccess) e = hipMalloc(&dg, gb);
+  if (e == hipSuccess) e = hipMalloc(&ws, ap.ws_bytes);
+  if (e == hipSuccess) e = hipMalloc(&dsc, sizeof(double) * (nc + 1));
+  auto cleanup = [&]() {
+    (void)hipFree(du0);
+    (void)hipFree(dt);
+    (void)hipFree(dg);
+    (void)hipFree(ws);
+    (void)hipFree(dsc);
+  };
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(AN5D_E_ALLOC, std::string("hipMalloc(adjoint): ") + hipGetErrorString(e));
+  }
+  e = hipMemcpy(du0, u0, gb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dt, target, gb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    cleanup();
+    return hip_fail(e, "adjoint H2D");
+  }
+  const double t0 = sb_time_ms();
+  double mf = 0.0;
+  rc = an5d_adjoint_device(p, du0, dt, dg, dsc, dsc + nc, ws, ap.ws_bytes, cfg, nullptr, &mf);
+  e = hipDeviceSynchronize();
+  const double t1 = sb_time_ms();
+  if (rc == AN5D_OK && e != hipSuccess) rc = hip_fail(e, "adjoint execution");
+  if (rc == AN5D_OK) {
+    e = hipMemcpy(grad, dg, gb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(gcoef, dsc, sizeof(double) * nc, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(loss, dsc + nc, sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "adjoint D2H");
+  }
+  cleanup();
+  if (ms_fwd) *ms_fwd = mf;
+  if (ms_total) *ms_total = t1 - t0;
+  return rc;
+}

@@ -1,0 +1,377 @@
+// Temporally blocked adjoint sweep (2D), hand-written HIP for gfx950.
+//
+// Problem: u_{t+1} = S(u_t) on the interior (halo fixed), loss
+// L = 1/2 sum_interior (u_T - target)^2.  Reverse mode over one time block
+// [t0, t0+BT): given the forward checkpoint u_{t0} and lambda_{t0+BT}
+// (= dL/du_{t0+BT} on the interior, zero on the halo), one launch
+//   * recomputes u_{t0+1..t0+BT-1} from the checkpoint (forward stages, the
+//     exact forward expression, so the states equal the forward pass),
+//   * runs BT adjoint stages lambda_t = J_t^T mu_{t+1}, mu = lambda masked to
+//     the interior:  linear stencils: lambda_t(y) = sum_q s_q c_q/div *
+//     mu_{t+1}(y - d_q) (the transposed taps);  gradient2d: the Jacobian of
+//     C + scale/sqrt(eps + sum_m (C - N_m)^2) at u_t, transposed,
+//   * accumulates the coefficient gradient sum_x mu_{t+1}(x) dF/dc(x, u_t)
+//     for every t of the block over the cells this wave owns.
+// All of it streams along dim 0 in registers like the forward sweep: forward
+// stage m lags the checkpoint input by m*R planes; the adjoint pipeline lags
+// it by DLT + a*R planes so that adjoint stage a (time t0+BT-a) finds the
+// forward state u_{t0+BT-a} it needs (W planes either side) still in the
+// stage's ring.  Fastest-dim neighbours come from DPP as in the forward.
+// Tolerance-based (FP32 vs an FP64 oracle): adjoint arithmetic contracts to
+// FMA; the recomputed forward states are exact.
+#pragma once
+
+#include "sweep.hpp"
+
+namespace an5d {
+
+template <typename T>
+struct AdjArgs {
+  const T* __restrict__ U;     // checkpoint u_{t0}
+  const T* __restrict__ Lin;   // lambda_{t0+BT}, zero on the halo
+  T* __restrict__ Lout;        // lambda_{t0}: interior written
+  const T* __restrict__ coef;  // forward coefficients, then the divisor (gradient2d: eps)
+  double gscale;
+  int has_div;
+  double* __restrict__ gacc;   // coefficient-gradient accumulators
+  int D0, D2;
+  long long s0;                // row pitch of U, Lin, Lout (elements)
+  int ntk, nseg, sl, SK;
+};
+
+template <class D>
+constexpr int adj_reach() { return D::linear ? D::rad : 2; }
+
+template <int BT, int R, int W>
+constexpr int adj_halo() { return (2 * BT - 2) * R + W; }  // garbage spread along the fastest dim
+
+template <typename T, int CK, int XK>
+__device__ __forceinline__ void extend_row(const T (&src)[CK], T (&x)[CK + 2 * XK]) {
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) x[kk + XK] = src[kk];
+#pragma unroll
+  for (int m = 1; m <= XK; m++) {
+    x[XK - m] = from_left(src[CK - m]);
+    x[XK + CK - 1 + m] = from_right(src[m - 1]);
+  }
+}
+
+// Stage input getter over row-extended planes (2D): plane A0 relative to the
+// centre, fastest-dim offset A2.
+template <class D, typename T, int NP, int CK, int XK, int KK>
+struct RowGet {
+  const T (&x)[NP][CK + 2 * XK];
+  int centre;
+  template <int A0, int A1, int A2>
+  __device__ __forceinline__ T at() const { return x[D::rad + A0][KK + A2 + XK]; }
+};
+
+template <typename T>
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int ID, typename T, int BT, int CK, int NW>
+__global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
+  using D = Def<ID>;
+  static_assert(D::dim == 2, "adjoint sweep is 2D");
+  constexpr int R = D::rad;
+  constexpr int P = 2 * R + 1;
+  constexpr int W = adj_reach<D>();
+  constexpr int DLT = W + (BT - 2) * R;  // adjoint lag behind the checkpoint input
+  constexpr int HU = 2 * (BT - 1) * R + W;
+  constexpr int HA = adj_halo<BT, R, W>();
+  constexpr int NT = D::linear ? D::ntaps : 1;
+  constexpr int NG = D::linear ? D::ntaps : 2;  // gradient entries
+  constexpr int XK = R;                          // fastest-dim extension of a plane
+  constexpr int XU = D::linear ? R : 1;          // fastest-dim reach into forward states
+  static_assert(CK >= R && CK >= XU, "a lane must hold at least R cells");
+  static_assert(DLT >= 0, "bad lag");
+  // forward ring of stage m holds LF(m) planes
+  constexpr auto LF = [](int m) constexpr { return 2 * W + (2 * BT - 2 * m - 2) * R + 1; };
+  constexpr int LFMAX = LF(0);
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = blockIdx.x * NW + w;
+  if (gw >= a.ntk * a.nseg) return;
+  const int tk = gw % a.ntk, seg = gw / a.ntk;
+  const int kv0 = R + tk * a.SK;
+  const int kv1 = min(kv0 + a.SK, a.D2 - R);
+  const int kl = kv0 - HA + lane * CK;
+  const int i0 = R + seg * a.sl;
+  const int sle = min(a.sl, a.D0 - R - i0);
+  const int nit = sle + HU + DLT + BT * R;
+
+  bool kin[CK], kst[CK];
+  bool kst_all = true;
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) {
+    const int k = kl + kk;
+    kin[kk] = k >= R && k < a.D2 - R;
+    kst[kk] = k >= kv0 && k < kv1;
+    kst_all = kst_all && kst[kk];
+  }
+  const bool kfull = kl >= 0 && kl + CK <= a.D2;
+
+  T c[NT + 1];
+#pragma unroll
+  for (int q = 0; q < NT + 1; q++) c[q] = a.coef[q];
+  const T div = c[NT];
+  const bool has_div = a.has_div != 0;
+  const double gscale = a.gscale;
+  // transposed taps: s_q c_q / div
+  T ct[NT];
+  if constexpr (D::linear) {
+    static_for<0, NT>([&](auto qc) {
+      constexpr int Q = decltype(qc)::value;
+      constexpr TapDef tp = D::taps[Q];
+      T v = tp.sign > 0 ? c[Q] : -c[Q];
+      ct[Q] = has_div ? v / div : v;
+    });
+  }
+
+  T fr[BT][LFMAX][CK];  // forward rings (stage m uses the last LF(m) slots)
+  T ar[BT][P][CK];      // adjoint rings (stage a output; stage 0 = lambda input)
+  double gsum[NG];
+  T gpart[NG];
+#pragma unroll
+  for (int g = 0; g < NG; g++) { gsum[g] = 0.0; gpart[g] = 0; }
+  int since_flush = 0;
+
+  auto load_row = [&](const T* base, int p, T (&dst)[CK]) {
+    if (p < 0 || p >= a.D0) return false;
+    const T* row = base + (long long)p * a.s0 + kl;
+    if (kfull) {
+      load_run<T, CK>(dst, row);
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) dst[kk] = (kl + kk >= 0 && kl + kk < a.D2) ? row[kk] : T(0);
+    }
+    return true;
+  };
+
+  for (int n = 0; n < nit; n++) {
+    const int pU = i0 - HU + n;
+    // ---- forward recompute ----------------------------------------------
+#pragma unroll
+    for (int p = 0; p < LFMAX - 1; p++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) fr[0][p][kk] = fr[0][p + 1][kk];
+    load_row(a.U, pU, fr[0][LFMAX - 1]);
+    static_for<1, BT>([&](auto mc) {
+      constexpr int M = decltype(mc)::value;
+      constexpr int LP = LF(M - 1);     // input ring length
+      constexpr int B0 = LFMAX - LP;    // first used slot of the input ring
+      const int q = pU - M * R;
+      T xin[P][CK + 2 * XK];
+      static_for<0, P>([&](auto pc) {
+        constexpr int PP = decltype(pc)::value;
+        extend_row<T, CK, XK>(fr[M - 1][B0 + LP - P + PP], xin[PP]);
+      });
+      const bool pin = q >= R && q < a.D0 - R;
+      T out[CK];
+      static_for<0, CK>([&](auto kc) {
+        constexpr int KK = decltype(kc)::value;
+        RowGet<D, T, P, CK, XK, KK> get{xin, R};
+        const T val = eval_point<D, T>(get, c, div, has_div, gscale);
+        out[KK] = (pin && kin[KK]) ? val : xin[R][KK + XK];
+      });
+#pragma unroll
+      for (int p = LFMAX - LF(M); p < LFMAX - 1; p++)
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) fr[M][p][kk] = fr[M][p + 1][kk];
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) fr[M][LFMAX - 1][kk] = out[kk];
+    });
+    // ---- adjoint stages --------------------------------------------------
+#pragma unroll
+    for (int p = 0; p < P - 1; p++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) ar[0][p][kk] = ar[0][p + 1][kk];
+    if (!load_row(a.Lin, pU - DLT, ar[0][P - 1])) {
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) ar[0][P - 1][kk] = T(0);
+    }
+    static_for<1, BT + 1>([&](auto ac) {
+      constexpr int A = decltype(ac)::value;
+      constexpr int M = BT - A;          // forward stage paired with this adjoint stage
+      constexpr int LM = LF(M);
+      constexpr int BM = LFMAX - LM;     // slot of plane qa - W in ring M
+      const int qa = pU - DLT - A * R;
+      const bool pin = qa >= R && qa < a.D0 - R;
+      const bool own = qa >= i0 && qa < i0 + sle;
+      // mu planes qa-R..qa+R, row-extended
+      T mx[P][CK + 2 * XK];
+      static_for<0, P>([&](auto pc) {
+        constexpr int PP = decltype(pc)::value;
+        extend_row<T, CK, XK>(ar[A - 1][PP], mx[PP]);
+      });
+      T out[CK];
+      if constexpr (D::linear) {
+#pragma clang fp contract(fast)
+        static_for<0, CK>([&](auto kc) {
+          constexpr int KK = decltype(kc)::value;
+          T acc = T(0);
+          static_for<0, NT>([&](auto qc) {
+            constexpr int Q = decltype(qc)::value;
+            constexpr TapDef tp = D::taps[Q];
+            acc = acc + ct[Q] * mx[R - tp.d0][KK - tp.d2 + XK];
+          });
+          out[KK] = (pin && kin[KK]) ? acc : T(0);
+        });
+        if (own) {
+          // sum_x mu(x) u_t(x + d_q) over owned cells of plane qa
+          T ux[2 * R + 1][CK + 2 * XK];
+          static_for<0, 2 * R + 1>([&](auto pc) {
+            constexpr int PP = decltype(pc)::value;
+            extend_row<T, CK, XK>(fr[M][BM + (W - R) + PP], ux[PP]);
+          });
+          static_for<0, CK>([&](auto kc) {
+            constexpr int KK = decltype(kc)::value;
+            const T mu = kst[KK] ? mx[R][KK + XK] : T(0);
+            static_for<0, NT>([&](auto qc) {
+              constexpr int Q = decltype(qc)::value;
+              constexpr TapDef tp = D::taps[Q];
+              gpart[Q] = gpart[Q] + mu * ux[R + tp.d0][KK + tp.d2 + XK];
+            });
+          });
+        }
+      } else {
+        // gradient2d: F(x) = C + scale * rs(x), rs = (eps + sum_m (C - N_m)^2)^-1/2
+        //   dF/dC = 1 + f * sum_m (C - N_m), dF/dN_m = -f (C - N_m), f = -scale rs^3
+        // lambda(y) = mu(y) dF/dC(y) + sum_m mu(y - e_m) dF/dN_m(y - e_m)
+#pragma clang fp contract(fast)
+        T ux[5][CK + 2 * XU];
+        static_for<0, 5>([&](auto pc) {
+          constexpr int PP = decltype(pc)::value;
+          extend_row<T, CK, XU>(fr[M][BM + PP], ux[PP]);
+        });
+        const T eps = c[0];
+        const T sc = (T)gscale;
+        // per plane qa-1, qa, qa+1 (index 1..3 in ux): f and the four differences
+        T fN[CK], fS[CK], gE[CK], gW[CK], gC[CK];
+        static_for<0, CK>([&](auto kc) {
+          constexpr int KK = decltype(kc)::value;
+          constexpr int K = KK + XU;
+          // x on plane qa+1 (ux row 3): contributes dF/dN (its N neighbour is y)
+          {
+            const T cc = ux[3][K];
+            const T dn = cc - ux[2][K], ds = cc - ux[4][K], de = cc - ux[3][K + 1], dw = cc - ux[3][K - 1];
+            const T s = eps + dn * dn + ds * ds + de * de + dw * dw;
+            const T rs = T(1) / __builtin_sqrt(s);
+            const T f = -sc * rs * rs * rs;
+            fN[KK] = mx[R + 1][KK + XK] * (-f * dn);
+          }
+          // x on plane qa-1 (ux row 1): its S neighbour is y
+          {
+            const T cc = ux[1][K];
+            const T dn = cc - ux[0][K], ds = cc - ux[2][K], de = cc - ux[1][K + 1], dw = cc - ux[1][K - 1];
+            const T s = eps + dn * dn + ds * ds + de * de + dw * dw;
+            const T rs = T(1) / __builtin_sqrt(s);
+            const T f = -sc * rs * rs * rs;
+            fS[KK] = mx[R - 1][KK + XK] * (-f * ds);
+          }
+          // x on plane qa: own centre term and the E/W fluxes to the neighbours
+          {
+            const T cc = ux[2][K];
+            const T dn = cc - ux[1][K], ds = cc - ux[3][K], de = cc - ux[2][K + 1], dw = cc - ux[2][K - 1];
+            const T s = eps + dn * dn + ds * ds + de * de + dw * dw;
+            const T rs = T(1) / __builtin_sqrt(s);
+            const T f = -sc * rs * rs * rs;
+            const T mu = mx[R][KK + XK];
+            gC[KK] = mu * (T(1) + f * (dn + ds + de + dw));
+            gE[KK] = mu * (-f * de);  // to x + (0,1)
+            gW[KK] = mu * (-f * dw);  // to x - (0,1)
+            if (own) {
+              const T muo = kst[KK] ? mu : T(0);
+              gpart[0] = gpart[0] + muo * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
+              gpart[1] = gpart[1] + muo * rs;                  // d/d scale
+            }
+          }
+        });
+        // y gets gE from x = y - (0,1) and gW from x = y + (0,1)
+        T gEx[CK + 2], gWx[CK + 2];
+        extend_row<T, CK, 1>(gE, gEx);
+        extend_row<T, CK, 1>(gW, gWx);
+        static_for<0, CK>([&](auto kc) {
+          constexpr int KK = decltype(kc)::value;
+          const T v = gC[KK] + fN[KK] + fS[KK] + gEx[KK] + gWx[KK + 2];
+          out[KK] = (pin && kin[KK]) ? v : T(0);
+        });
+      }
+      if constexpr (A < BT) {
+#pragma unroll
+        for (int p = 0; p < P - 1; p++)
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) ar[A][p][kk] = ar[A][p + 1][kk];
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) ar[A][P - 1][kk] = out[kk];
+      } else {
+        if (own) {
+          T* row = a.Lout + (long long)qa * a.s0 + kl;
+          if (kst_all) {
+            store_run<T, CK>(row, out);
+          } else {
+#pragma unroll
+            for (int kk = 0; kk < CK; kk++)
+              if (kst[kk]) row[kk] = out[kk];
+          }
+        }
+      }
+    });
+    if (++since_flush == 16) {
+#pragma unroll
+      for (int g = 0; g < NG; g++) { gsum[g] += (double)gpart[g]; gpart[g] = 0; }
+      since_flush = 0;
+    }
+  }
+  static_for<0, NG>([&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    double v = wave_sum<T>(gsum[G] + (double)gpart[G]);
+    if constexpr (D::linear) {
+      // d u_{t+1} / d c_q = s_q u_t(x + d_q) / div
+      v = (D::taps[G].sign > 0 ? v : -v) / (has_div ? (double)div : 1.0);
+    }
+    if (lane == 0) atomicAdd(a.gacc + G, v);
+  });
+}
+
+// lambda_T = (u_T - target) on the interior, 0 on the halo; loss partial sums.
+template <typename T>
+__global__ __launch_bounds__(256) void adjoint_seed_kernel(const T* __restrict__ uT,
+                                                           const T* __restrict__ target,
+                                                           T* __restrict__ lam, double* __restrict__ loss,
+                                                           int D0, int D2, long long s0, int R) {
+  const long long total = (long long)D0 * D2;
+  double part = 0.0;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / D2), k = (int)(idx % D2);
+    const long long x = (long long)i * s0 + k;
+    const bool in = i >= R && i < D0 - R && k >= R && k < D2 - R;
+    T d = in ? uT[x] - target[x] : T(0);
+    lam[x] = d;
+    part += 0.5 * (double)d * (double)d;
+  }
+  part = wave_sum<T>(part);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, part);
+}
+
+// Copy the halo cells of `src` into `dst` (same layout), for the checkpoint slots.
+template <typename T>
+__global__ __launch_bounds__(256) void halo_copy_kernel(const T* __restrict__ src, T* __restrict__ dst,
+                                                        int D0, int D2, long long s0, int R) {
+  const long long total = (long long)D0 * D2;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / D2), k = (int)(idx % D2);
+    if (i >= R && i < D0 - R && k >= R && k < D2 - R) continue;
+    const long long x = (long long)i * s0 + k;
+    dst[x] = src[x];
+  }
+}
+
+}  // namespace an5d

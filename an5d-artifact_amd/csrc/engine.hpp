@@ -39,4 +39,33 @@ struct Variant {
 // Every stencil x dtype translation unit appends its variants here.
 std::vector<Variant>& registry();
 
+// dtype-erased AdjArgs<T>
+struct AdjLaunch {
+  const void* U;
+  const void* Lin;
+  void* Lout;
+  const void* coef;
+  double gscale;
+  int has_div;
+  double* gacc;
+  int D0, D2;
+  long long s0;
+  int ntk, nseg, sl, SK;
+};
+using AdjFn = hipError_t (*)(const AdjLaunch&, dim3 grid, dim3 block, hipStream_t);
+using SeedFn = hipError_t (*)(const void* uT, const void* target, void* lam, double* loss, int D0,
+                              int D2, long long s0, int R, hipStream_t);
+using HaloFn = hipError_t (*)(const void* src, void* dst, int D0, int D2, long long s0, int R,
+                              hipStream_t);
+
+struct AdjVariant {
+  int stencil, dtype;
+  int ck, nw, btmax, reach;
+  SeedFn seed;
+  HaloFn halo;
+  AdjFn fn[kMaxBT + 1];
+};
+
+std::vector<AdjVariant>& adj_registry();
+
 }  // namespace an5d

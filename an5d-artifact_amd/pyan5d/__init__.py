@@ -75,6 +75,13 @@ def lib():
                                   ctypes.c_int, ctypes.c_void_p]
         L.an5d_init_grid.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
                                      ctypes.c_int]
+        L.an5d_adjoint_host.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        P(ctypes.c_double), P(ctypes.c_double), P(Config),
+                                        P(ctypes.c_double), P(ctypes.c_double)]
+        L.an5d_adjoint_workspace.argtypes = [P(Problem), P(Config), P(ctypes.c_size_t)]
+        L.an5d_adjoint_device.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                          P(Config), ctypes.c_void_p, P(ctypes.c_double)]
         L.an5d_cpu_forward.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double)]
         _lib = L
     return _lib
@@ -251,3 +258,52 @@ def _np_dtype(t):
     if t.dtype == torch.float64:
         return np.float64
     raise ValueError(f"unsupported tensor dtype {t.dtype}")
+
+
+def adjoint_host(name, u0, target, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0):
+    """Loss 1/2||u_T - target||^2 (interior), dL/du0 (interior; halo 0) and
+    dL/dcoefs on the GPU.  u0, target: single grids (halo included).
+    Returns (loss, grad, gcoef, ms_fwd, ms_total)."""
+    u0 = np.ascontiguousarray(u0)
+    target = np.ascontiguousarray(target, dtype=u0.dtype)
+    d = info(name)
+    p, keep = make_problem(name, n, T, u0.dtype, coefs, divisor)
+    cfg = make_config(bt, variant, sl)
+    grad = np.zeros_like(u0)
+    nc = d["ntaps"] if d["linear"] else 2
+    gc = np.zeros(nc, dtype=np.float64)
+    loss = ctypes.c_double()
+    mf, mt = ctypes.c_double(), ctypes.c_double()
+    _check(lib().an5d_adjoint_host(ctypes.byref(p), u0.ctypes.data, target.ctypes.data, grad.ctypes.data,
+                                   gc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(loss),
+                                   ctypes.byref(cfg), ctypes.byref(mf), ctypes.byref(mt)))
+    del keep
+    return loss.value, grad, gc, mf.value, mt.value
+
+
+class AdjointWorkspace:
+    """Device workspace for adjoint_device (checkpoints + two adjoint buffers)."""
+
+    def __init__(self, name, n, T, dtype, bt=0, variant=-1, sl=0):
+        import torch
+        p, _ = make_problem(name, n, T, dtype)
+        cfg = make_config(bt, variant, sl)
+        nbytes = ctypes.c_size_t()
+        _check(lib().an5d_adjoint_workspace(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(nbytes)))
+        self.nbytes = nbytes.value
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+
+
+def adjoint_device(name, u0, target, grad, gcoef, loss, n, T, ws, coefs=None, divisor=-1.0,
+                   bt=0, variant=-1, sl=0, stream=None, time_forward=False):
+    """Device tensors: u0/target/grad single grids, gcoef/loss float64 tensors."""
+    p, keep = make_problem(name, n, T, _np_dtype(u0), coefs, divisor)
+    cfg = make_config(bt, variant, sl)
+    mf = ctypes.c_double()
+    _check(lib().an5d_adjoint_device(ctypes.byref(p), ctypes.c_void_p(u0.data_ptr()),
+                                     ctypes.c_void_p(target.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
+                                     ctypes.c_void_p(gcoef.data_ptr()), ctypes.c_void_p(loss.data_ptr()),
+                                     ctypes.c_void_p(ws.buf.data_ptr()), ws.nbytes, ctypes.byref(cfg),
+                                     _stream_ptr(stream), ctypes.byref(mf) if time_forward else None))
+    del keep
+    return mf.value if time_forward else None

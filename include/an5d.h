@@ -138,6 +138,27 @@ int an5d_cpu_forward(const an5d_problem* prob, void* A, int threads, double* ms)
  * uniform in [0, 1). */
 int an5d_init_grid(int dtype, void* A, long long cells, long long seed_base, int mode);
 
+/* ---- adjoint (2D stencils) ------------------------------------------------
+ * No counterpart in the reference, which computes no gradients.
+ * Loss L = 1/2 sum over interior cells of (u_T - target)^2.  Outputs:
+ * dL/du_0 on the interior (the halo is a fixed boundary condition: its
+ * entries are 0), dL/dcoefs (one per tap in source order; gradient2d:
+ * {d/d eps, d/d scale}) and L.  Single grids (one buffer, D0 x D1 cells,
+ * reference layout).  The forward pass keeps a checkpoint at every launch
+ * boundary (bt steps); each reverse launch recomputes the states of its
+ * block from the checkpoint and runs the transposed sweep for bt steps. */
+int an5d_adjoint_workspace(const an5d_problem* prob, const an5d_config* cfg, size_t* bytes);
+/* Device form: enqueues on `stream`; *ms_fwd (optional) gets the time of the
+ * checkpointed forward pass (synchronises on an event when non-NULL).
+ * gcoef and loss are device doubles. */
+int an5d_adjoint_device(const an5d_problem* prob, const void* u0, const void* target, void* grad,
+                        double* gcoef, double* loss, void* workspace, size_t workspace_bytes,
+                        const an5d_config* cfg, void* stream, double* ms_fwd);
+/* Host form: copies in, runs, copies out; *ms_total = forward + adjoint. */
+int an5d_adjoint_host(const an5d_problem* prob, const void* u0, const void* target, void* grad,
+                      double* gcoef, double* loss, const an5d_config* cfg, double* ms_fwd,
+                      double* ms_total);
+
 /* Release cached device buffers. */
 int an5d_release(void);
 

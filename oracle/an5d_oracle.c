@@ -269,17 +269,14 @@ DEFINE_CHECK(double, f64)
  * FP64 adjoint oracle (no reference counterpart).
  *
  * Forward: u_{t+1} = S(u_t) on the interior, halo fixed.  Loss
- * L = 0.5*||u_T - target||^2 over the whole buffer (the halo term is
- * constant).  With lam_T = u_T - target on the interior (0 on the halo):
- *   linear:    lam_t(y) = sum_taps c_q/div * lam_{t+1}(y - d_q), y interior
- *                         (lam on halo cells is accumulated into gh, the
- *                         gradient w.r.t. the fixed halo values, and not
- *                         propagated further)
- *              dL/dc_q += sum_x lam_{t+1}(x) * u_t(x + d_q) / div
- *   gradient2d:J_t from u_t (recomputed by replaying the forward).
- * gu (out) = dL/du_0 over the whole grid: interior = lam_0, halo = the
- * accumulated halo sensitivities plus the direct loss term on the halo.
- * Implemented by storing all T states (oracle sizes only).
+ * L = 0.5 * sum over interior cells of (u_T - target)^2.  The halo is a
+ * boundary condition, not a variable: its gradient entries are 0.
+ * With lam_T = u_T - target on the interior (0 on the halo):
+ *   linear:    lam_t(y) = sum_taps s_q c_q/div * lam_{t+1}(y - d_q), y interior
+ *              dL/dc_q += sum_x lam_{t+1}(x) * s_q u_t(x + d_q) / div
+ *   gradient2d: J_t^T from u_t (all states stored; oracle sizes only),
+ *              dL/deps += lam * (-scale/2) rs^3, dL/dscale += lam * rs.
+ * gu (out) = lam_0 on the interior, 0 on the halo.
  * ------------------------------------------------------------------------ */
 int oracle_adjoint_f64(int id, const double* u0, const double* target,
                        const int64_t* n, int T_steps, const double* coefs,
@@ -304,20 +301,17 @@ int oracle_adjoint_f64(int id, const double* u0, const double* target,
   double L = 0.0;
   double* lam = (double*)calloc((size_t)area, sizeof(double));
   double* nlam = (double*)calloc((size_t)area, sizeof(double));
-  for (int64_t x = 0; x < area; x++) {
-    double d = uT[x] - target[x];
-    L += 0.5 * d * d;
-    gu[x] = 0.0;
-  }
-  /* interior of lam_T; halo part of the direct term goes straight to gu */
+  for (int64_t x = 0; x < area; x++) gu[x] = 0.0;
   for (int64_t i = 0; i < g.D0; i++)
     for (int64_t j = 0; j < g.D1; j++)
       for (int64_t k = 0; k < g.D2; k++) {
         int64_t x = i * s0 + j * s1 + k;
         int interior = i >= g.r && i < g.D0 - g.r && k >= g.r && k < g.D2 - g.r &&
                        (s->dim == 2 || (j >= g.r && j < g.D1 - g.r));
-        if (interior) lam[x] = uT[x] - target[x];
-        else gu[x] += uT[x] - target[x];
+        if (interior) {
+          lam[x] = uT[x] - target[x];
+          L += 0.5 * lam[x] * lam[x];
+        }
       }
   int nc = s->linear ? s->ntaps : 2;
   for (int q = 0; q < nc; q++) gc[q] = 0.0;
@@ -362,18 +356,18 @@ int oracle_adjoint_f64(int id, const double* u0, const double* target,
             gc[1] += l * rs;
           }
         }
-    /* lam on halo cells is a sensitivity to the fixed halo: collect it */
+    /* lam on halo cells would be a sensitivity to the fixed boundary: drop it */
     for (int64_t i = 0; i < g.D0; i++)
       for (int64_t j = 0; j < g.D1; j++)
         for (int64_t k = 0; k < g.D2; k++) {
           int64_t x = i * s0 + j * s1 + k;
           int interior = i >= g.r && i < g.D0 - g.r && k >= g.r && k < g.D2 - g.r &&
                          (s->dim == 2 || (j >= g.r && j < g.D1 - g.r));
-          if (!interior) { gu[x] += nlam[x]; nlam[x] = 0.0; }
+          if (!interior) nlam[x] = 0.0;
         }
     double* tmp = lam; lam = nlam; nlam = tmp;
   }
-  for (int64_t x = 0; x < area; x++) gu[x] += lam[x];
+  for (int64_t x = 0; x < area; x++) gu[x] = lam[x];
   *loss = L;
   free(c); free(lam); free(nlam); free(states); free(buf);
   return 0;

@@ -1,0 +1,105 @@
+"""GPU adjoint sweep against the FP64 oracle adjoint and central finite
+differences (FP64, 64x64), for every 2D stencil and block depth."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pyan5d
+
+pytestmark = pytest.mark.gpu
+
+NAMES_2D = [n for n in O.NAMES if O.info(n)["dim"] == 2]
+
+
+def problem(name, n, dt, seed):
+    d = O.info(name)
+    rng = np.random.default_rng(seed)
+    shape = tuple(x + 2 * d["rad"] for x in n)
+    u0 = rng.random(shape).astype(dt)
+    target = rng.random(shape).astype(dt)
+    if d["linear"]:
+        c = 1.0 - rng.random(d["ntaps"])
+        c = c / c.sum()
+        return u0, target, c, 0.0
+    return u0, target, np.array([1e-4, 0.01]), -1.0
+
+
+def cases():
+    out = []
+    for name in NAMES_2D:
+        for dt in (np.float32, np.float64):
+            adj_bt = {1: 4, 2: 2}.get(O.info(name)["rad"], 1)
+            if not O.info(name)["linear"]:
+                adj_bt = 4 if dt == np.float32 else 3
+            elif O.info(name)["rad"] == 1 and O.info(name)["ntaps"] > 5:
+                adj_bt = 3
+            for bt in sorted({1, adj_bt}):
+                out.append((name, dt, bt))
+    return out
+
+
+@pytest.mark.parametrize("name,dt,bt", cases())
+def test_adjoint_vs_oracle(name, dt, bt):
+    n = [37, 300]
+    T = 2 * bt + 1
+    u0, target, c, div = problem(name, n, dt, seed=bt)
+    print(f"case adjoint {name} {np.dtype(dt).name} bt={bt} T={T}", flush=True)
+    loss, g, gc, _, _ = pyan5d.adjoint_host(name, u0, target, n, T, coefs=c, divisor=div, bt=bt)
+    oloss, og, ogc = O.adjoint(name, u0.astype(np.float64), target.astype(np.float64), n, T,
+                               coefs=c, divisor=div)
+    tol = 1e-10 if dt == np.float64 else 2e-4
+    assert abs(loss - oloss) <= tol * max(1.0, abs(oloss))
+    assert np.max(np.abs(g - og)) <= tol * max(1e-30, np.max(np.abs(og)))
+    assert np.max(np.abs(gc - ogc)) <= tol * max(1e-30, np.max(np.abs(ogc)))
+
+
+@pytest.mark.parametrize("name", ["j2d9pt-gol", "j2d9pt", "gradient2d", "j2d5pt"])
+def test_adjoint_central_differences_fp64(name):
+    """FD on a 64x64 FP64 instance through the GPU forward."""
+    n, T = [64, 64], 12
+    u0, target, c, div = problem(name, n, np.float64, seed=7)
+    d = O.info(name)
+    r = d["rad"]
+
+    def loss_of(u, coefs):
+        A = np.stack([u, u]).copy()
+        pyan5d.forward_host(name, A, n, T, coefs=coefs, divisor=div)
+        return 0.5 * float(np.sum((A[T % 2][r:-r, r:-r] - target[r:-r, r:-r]) ** 2))
+
+    print(f"case fd {name}", flush=True)
+    loss, g, gc, _, _ = pyan5d.adjoint_host(name, u0, target, n, T, coefs=c, divisor=div)
+    assert abs(loss - loss_of(u0, c)) <= 1e-12 * loss
+    rng = np.random.default_rng(1)
+    eps = 1e-6
+    for _ in range(8):
+        x = (int(rng.integers(r, 64 + r)), int(rng.integers(r, 64 + r)))
+        up, um = u0.copy(), u0.copy()
+        up[x] += eps
+        um[x] -= eps
+        fd = (loss_of(up, c) - loss_of(um, c)) / (2 * eps)
+        assert abs(fd - g[x]) <= 1e-6 * max(1.0, abs(fd)), (x, fd, g[x])
+    for q in range(len(c)):
+        h = eps * max(1.0, abs(c[q]))
+        cp, cm = c.copy(), c.copy()
+        cp[q] += h
+        cm[q] -= h
+        fd = (loss_of(u0, cp) - loss_of(u0, cm)) / (2 * h)
+        assert abs(fd - gc[q]) <= 1e-5 * max(1.0, abs(fd)), (q, fd, gc[q])
+    assert np.all(g[:r, :] == 0) and np.all(g[:, -r:] == 0)
+
+
+def test_adjoint_device_entry(torch_cuda):
+    torch = torch_cuda
+    name, n, T = "j2d9pt-gol", [100, 700], 11
+    u0, target, c, div = problem(name, n, np.float32, seed=3)
+    ws = pyan5d.AdjointWorkspace(name, n, T, np.float32)
+    tu, tt = torch.from_numpy(u0).cuda(), torch.from_numpy(target).cuda()
+    tg = torch.zeros_like(tu)
+    gc = torch.zeros(9, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    print("case adjoint_device", flush=True)
+    pyan5d.adjoint_device(name, tu, tt, tg, gc, loss, n, T, ws, coefs=c, divisor=div)
+    torch.cuda.synchronize()
+    hl, hg, hgc, _, _ = pyan5d.adjoint_host(name, u0, target, n, T, coefs=c, divisor=div)
+    assert np.array_equal(tg.cpu().numpy(), hg)
+    assert np.allclose(gc.cpu().numpy(), hgc, rtol=1e-12, atol=0)

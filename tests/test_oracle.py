@@ -169,3 +169,45 @@ def test_check_result_restatement():
     assert abs(rms - np.sqrt(25.0 / 16)) < 1e-12
     mx0, _ = O.check(A, B, 2)
     assert mx0 == 0.0
+
+
+def _loss(name, u0, target, n, T, coefs=None):
+    A = np.stack([u0, u0]).astype(np.float64)
+    O.run(name, A, n, T, coefs=coefs)
+    d = O.info(name)
+    r = d["rad"]
+    sl = tuple([slice(r, -r)] * d["dim"])
+    return 0.5 * float(np.sum((A[T % 2][sl] - target[sl]) ** 2))
+
+
+@pytest.mark.parametrize("name", ["j2d5pt", "j2d9pt", "j2d9pt-gol", "box2d2r", "gradient2d", "star3d1r"])
+def test_oracle_adjoint_matches_finite_differences(name):
+    """Central differences of the oracle forward pin the oracle adjoint,
+    for the initial state and for every coefficient."""
+    d = O.info(name)
+    n = [9, 11] if d["dim"] == 2 else [5, 6, 7]
+    T = 4
+    rng = np.random.default_rng(3)
+    shape = tuple(x + 2 * d["rad"] for x in n)
+    u0 = rng.random(shape)
+    target = rng.random(shape)
+    coefs = (np.array([1e-2, 0.5]) if not d["linear"] else
+             np.asarray(O.taps(name)[1], dtype=np.float64))
+    loss, gu, gc = O.adjoint(name, u0, target, n, T, coefs=coefs)
+    assert abs(loss - _loss(name, u0, target, n, T, coefs)) < 1e-12 * max(1.0, loss)
+    eps = 1e-6
+    r = d["rad"]
+    for _ in range(6):
+        x = tuple(int(rng.integers(r, s - r)) for s in shape)
+        up, um = u0.copy(), u0.copy()
+        up[x] += eps
+        um[x] -= eps
+        fd = (_loss(name, up, target, n, T, coefs) - _loss(name, um, target, n, T, coefs)) / (2 * eps)
+        assert abs(fd - gu[x]) <= 1e-6 * max(1.0, abs(fd)), (x, fd, gu[x])
+    for q in range(len(coefs)):
+        cp, cm = coefs.copy(), coefs.copy()
+        h = eps * max(1.0, abs(coefs[q]))
+        cp[q] += h
+        cm[q] -= h
+        fd = (_loss(name, u0, target, n, T, cp) - _loss(name, u0, target, n, T, cm)) / (2 * h)
+        assert abs(fd - gc[q]) <= 1e-5 * max(1.0, abs(fd)), (q, fd, gc[q])

@@ -116,11 +116,11 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   }
   const bool kfull = kl >= 0 && kl + CK <= a.D2;
 
-  T c[NT + 1];
+  T c[NT + 2];
 #pragma unroll
-  for (int q = 0; q < NT + 1; q++) c[q] = a.coef[q];
+  for (int q = 0; q < NT + 2; q++) c[q] = a.coef[q];
   const T div = c[NT];
-  const bool has_div = a.has_div != 0;
+  const int has_div = a.has_div;
   const double gscale = a.gscale;
   // transposed taps: s_q c_q / div
   T ct[NT];
@@ -133,8 +133,23 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
     });
   }
 
+  // Rings start at zero and out-of-grid planes load as zero: every value the
+  // pipeline computes is then finite, so the masked products (mu == 0 off
+  // the interior) stay exactly zero, also for gradient2d's weights.
   T fr[BT][LFMAX][CK];  // forward rings (stage m uses the last LF(m) slots)
   T ar[BT][P][CK];      // adjoint rings (stage a output; stage 0 = lambda input)
+#pragma unroll
+  for (int m = 0; m < BT; m++)
+#pragma unroll
+    for (int p = 0; p < LFMAX; p++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) fr[m][p][kk] = T(0);
+#pragma unroll
+  for (int m = 0; m < BT; m++)
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) ar[m][p][kk] = T(0);
   double gsum[NG];
   T gpart[NG];
 #pragma unroll
@@ -142,7 +157,11 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   int since_flush = 0;
 
   auto load_row = [&](const T* base, int p, T (&dst)[CK]) {
-    if (p < 0 || p >= a.D0) return false;
+    if (p < 0 || p >= a.D0) {
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) dst[kk] = T(0);
+      return false;
+    }
     const T* row = base + (long long)p * a.s0 + kl;
     if (kfull) {
       load_run<T, CK>(dst, row);
@@ -191,10 +210,7 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
     for (int p = 0; p < P - 1; p++)
 #pragma unroll
       for (int kk = 0; kk < CK; kk++) ar[0][p][kk] = ar[0][p + 1][kk];
-    if (!load_row(a.Lin, pU - DLT, ar[0][P - 1])) {
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) ar[0][P - 1][kk] = T(0);
-    }
+    load_row(a.Lin, pU - DLT, ar[0][P - 1]);
     static_for<1, BT + 1>([&](auto ac) {
       constexpr int A = decltype(ac)::value;
       constexpr int M = BT - A;          // forward stage paired with this adjoint stage

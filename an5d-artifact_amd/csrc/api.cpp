@@ -130,12 +130,12 @@ int default_sl(const Variant& v, int bt, const Geom& g, long long ntk, long long
 }
 
 bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp,
-                 std::string& err) {
+                 std::string& err, int mode) {
   const int R = g.r;
   const long long n0 = g.D0 - 2 * R, nj = g.D1 - (g.dim == 3 ? 2 * R : 0), nk = g.D2 - 2 * R;
   LaunchArgs& a = lp.args;
   a.D0 = (int)g.D0; a.D1 = (int)g.D1; a.D2 = (int)g.D2;
-  lp.fn = (bt >= 1 && bt <= v.btmax) ? v.fn[bt] : nullptr;
+  lp.fn = (bt >= 1 && bt <= v.btmax) ? (mode == AN5D_FAST ? v.fn_fast[bt] : v.fn[bt]) : nullptr;
   if (!lp.fn) { err = "no kernel for bt=" + std::to_string(bt) + " in this variant"; return false; }
   lp.lds = v.lds;
   if (v.kind == kDirect) {
@@ -167,33 +167,43 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
 }
 
 // ---- coefficient buffers -------------------------------------------------------
-// Host image of the kernel's coefficient array: ntaps values (T), then the
-// divisor; for gradient2d {eps, 0, 0}.
+// Host image of the kernel's coefficient array: ntaps values (T), the
+// divisor and RN(1/divisor); for gradient2d {eps, 0, 0}.  has_div: 0 none,
+// 1 IEEE division, 2 the corrected reciprocal product (only for the source
+// divisors 118 and 159, for which it was checked against IEEE division).
 bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,
-                 double& gscale, std::string& err) {
+                 double& gscale, std::string& err, int mode) {
   const StencilRT& s = stencil_rt(p->stencil);
   const bool f64 = p->dtype == AN5D_FLOAT64;
   const int nt = s.linear ? s.ntaps : 1;
-  std::vector<double> vals(nt + 1, 0.0);
+  std::vector<double> vals(nt + 2, 0.0);
   gscale = 1.0;
   has_div = 0;
   if (s.linear) {
     for (int q = 0; q < s.ntaps; q++)
       vals[q] = p->coefs ? p->coefs[q] : (double)s.taps[q].coef;
-    if (p->divisor < 0) {
-      has_div = s.divisor != 0;
-      vals[nt] = s.divisor;
-    } else if (p->divisor > 0) {
-      has_div = 1;
-      vals[nt] = p->divisor;
+    double d = 0.0;
+    if (p->divisor < 0) d = s.divisor;
+    else if (p->divisor > 0) d = p->divisor;
+    if (d != 0.0) {
+      if (!(d == d) || d == INFINITY) { err = "divisor must be finite"; return false; }
+      has_div = (d == 118.0 || d == 159.0) ? 2 : 1;
+      vals[nt] = d;
     }
   } else {
     vals[0] = p->coefs ? p->coefs[0] : (double)0.0001f;
     gscale = p->coefs ? p->coefs[1] : 1.0;
     if (!(gscale == gscale)) { err = "scale is NaN"; return false; }
   }
+  if (mode == AN5D_FAST && s.linear) {
+    // fold signs and the divisor into the coefficients: sum_q fma(c'_q, v_q, .)
+    const double d = has_div ? vals[nt] : 1.0;
+    for (int q = 0; q < s.ntaps; q++) vals[q] = (s.taps[q].sign > 0 ? vals[q] : -vals[q]) / d;
+    has_div = 0;
+    vals[nt] = 0.0;
+  }
   const size_t esz = f64 ? 8 : 4;
-  bytes.resize((nt + 1) * esz);
+  bytes.resize((nt + 2) * esz);
   for (int q = 0; q <= nt; q++) {
     if (f64) {
       double v = vals[q];
@@ -202,6 +212,13 @@ bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& 
       float v = (float)vals[q];
       memcpy(&bytes[q * 4], &v, 4);
     }
+  }
+  if (f64) {
+    double rc = has_div ? 1.0 / vals[nt] : 0.0;
+    memcpy(&bytes[(nt + 1) * 8], &rc, 8);
+  } else {
+    float rc = has_div ? 1.0f / (float)vals[nt] : 0.0f;
+    memcpy(&bytes[(nt + 1) * 4], &rc, 4);
   }
   return true;
 }
@@ -400,7 +417,9 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
   std::vector<unsigned char> cbytes;
   int has_div = 0;
   double gscale = 1.0;
-  if (!build_coefs(p, cbytes, has_div, gscale, err)) return fail(AN5D_E_INVALID, err);
+  const int mode = cfg ? cfg->mode : AN5D_EXACT;
+  if (mode != AN5D_EXACT && mode != AN5D_FAST) return fail(AN5D_E_INVALID, "bad mode");
+  if (!build_coefs(p, cbytes, has_div, gscale, err, mode)) return fail(AN5D_E_INVALID, err);
   const void* dcoef = nullptr;
   rc = device_coefs(cbytes, &dcoef);
   if (rc) return rc;
@@ -418,7 +437,7 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
   int c0 = c0_first;
   for (int k : steps) {
     LaunchPlan lp;
-    if (!plan_launch(*v, k, g, sl, lp, err)) return fail(AN5D_E_UNSUPPORTED, err);
+    if (!plan_launch(*v, k, g, sl, lp, err, mode)) return fail(AN5D_E_UNSUPPORTED, err);
     char* base = static_cast<char*>(dA);
     lp.args.src = base + (size_t)(c0 % 2) * L.buf_stride * esz;
     lp.args.dst = base + (size_t)((c0 + 1) % 2) * L.buf_stride * esz;
@@ -513,10 +532,11 @@ extern "C" int an5d_kernel_stencil(int stencil, int dtype, void* A1, int compsiz
   if ((long long)compsize + 2 * s.rad < 2 * s.rad + 1 || timestep < 1) return AN5D_OK;
   if (scop) {
     // tiling overrides for run drivers: AN5D_BT, AN5D_VARIANT, AN5D_SL
-    an5d_config cfg{0, -1, 0};
+    an5d_config cfg{0, -1, 0, AN5D_EXACT};
     if (const char* e = getenv("AN5D_BT")) cfg.bt = atoi(e);
     if (const char* e = getenv("AN5D_VARIANT")) cfg.variant = atoi(e);
     if (const char* e = getenv("AN5D_SL")) cfg.sl = atoi(e);
+    if (const char* e = getenv("AN5D_MODE")) cfg.mode = atoi(e);
     return an5d_forward_host(&p, A1, &cfg, ms);
   }
   return an5d_cpu_forward(&p, A1, 0, ms);

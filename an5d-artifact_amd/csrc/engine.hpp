@@ -33,7 +33,8 @@ struct Variant {
   int stencil, dtype, kind;
   int ck, cj, nw, btmax;
   size_t lds;              // dynamic LDS bytes per workgroup
-  LaunchFn fn[kMaxBT + 1];  // fn[bt], bt in [1, btmax]
+  LaunchFn fn[kMaxBT + 1];       // fn[bt], bt in [1, btmax], exact mode
+  LaunchFn fn_fast[kMaxBT + 1];  // fast (FMA) mode, same shapes
 };
 
 // Every stencil x dtype translation unit appends its variants here.

@@ -30,16 +30,17 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
   return s;
 }
 
-template <int BT, int CK, int CJ, int NW>
+template <int BT, int CK, int CJ, int NW, bool EXACT>
 hipError_t launch_sweep(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
   if (lds > 65536) {
     // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default once
     static const hipError_t attr = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW>),
+        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (attr != hipSuccess) return attr;
   }
-  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW>), grid, block, lds, st, to_args<T>(a));
+  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>), grid, block, lds, st,
+                     to_args<T>(a));
   return hipGetLastError();
 }
 
@@ -61,7 +62,8 @@ void add_sweep(std::vector<Variant>& out) {
     v.lds = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>();
     static_for<1, sh.btmax + 1>([&](auto bc) {
       constexpr int BT = decltype(bc)::value;
-      v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw>;
+      v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true>;
+      v.fn_fast[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, false>;
     });
     out.push_back(v);
   }
@@ -82,6 +84,7 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   d.ck = direct_ck<SID, T>(); d.cj = 1; d.nw = 4; d.btmax = 1;
   d.lds = 0;
   d.fn[1] = &launch_direct<direct_ck<SID, T>()>;
+  d.fn_fast[1] = d.fn[1];
   out.push_back(d);
 }
 

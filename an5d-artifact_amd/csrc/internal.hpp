@@ -41,9 +41,9 @@ bool make_geom(const an5d_problem* p, Geom& g, std::string& err);
 an5d_layout reference_layout(const Geom& g);
 an5d_layout padded_layout(const Geom& g);
 bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp,
-                 std::string& err);
+                 std::string& err, int mode = 0);
 bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,
-                 double& gscale, std::string& err);
+                 double& gscale, std::string& err, int mode = 0);
 int device_coefs(const std::vector<unsigned char>& bytes, const void** out);
 int device_grid(size_t bytes, void** out);
 int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,

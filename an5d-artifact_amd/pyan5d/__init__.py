@@ -31,7 +31,11 @@ class Problem(ctypes.Structure):
 
 
 class Config(ctypes.Structure):
-    _fields_ = [("bt", ctypes.c_int), ("variant", ctypes.c_int), ("sl", ctypes.c_int)]
+    _fields_ = [("bt", ctypes.c_int), ("variant", ctypes.c_int), ("sl", ctypes.c_int),
+                ("mode", ctypes.c_int)]
+
+
+EXACT, FAST = 0, 1
 
 
 class Layout(ctypes.Structure):
@@ -163,17 +167,17 @@ def make_problem(name, n, T, dtype, coefs=None, divisor=-1.0):
     return p, keep
 
 
-def make_config(bt=0, variant=-1, sl=0):
+def make_config(bt=0, variant=-1, sl=0, mode=EXACT):
     c = Config()
-    c.bt, c.variant, c.sl = int(bt), int(variant), int(sl)
+    c.bt, c.variant, c.sl, c.mode = int(bt), int(variant), int(sl), int(mode)
     return c
 
 
-def forward_host(name, A, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0):
+def forward_host(name, A, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, mode=EXACT):
     """GPU forward on a host numpy array A[2, ...] (reference layout), in place; returns ms."""
     assert A.flags.c_contiguous
     p, keep = make_problem(name, n, T, A.dtype, coefs, divisor)
-    cfg = make_config(bt, variant, sl)
+    cfg = make_config(bt, variant, sl, mode)
     ms = ctypes.c_double()
     _check(lib().an5d_forward_host(ctypes.byref(p), A.ctypes.data, ctypes.byref(cfg), ctypes.byref(ms)))
     del keep
@@ -229,22 +233,24 @@ def layout_of(t, dim):
     return L
 
 
-def forward_device(name, t, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, stream=None):
+def forward_device(name, t, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, stream=None,
+                   mode=EXACT):
     """Enqueue T steps on a device tensor t[2, ...]; returns immediately."""
     d = info(name)
     p, keep = make_problem(name, n, T, _np_dtype(t), coefs, divisor)
-    cfg = make_config(bt, variant, sl)
+    cfg = make_config(bt, variant, sl, mode)
     L = layout_of(t, d["dim"])
     _check(lib().an5d_forward_device(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()), ctypes.byref(L),
                                      ctypes.byref(cfg), _stream_ptr(stream)))
     del keep
 
 
-def launch(name, t, n, steps, c0=0, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, stream=None):
+def launch(name, t, n, steps, c0=0, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, stream=None,
+           mode=EXACT):
     """One launch of `steps` fused steps from buffer c0%2 to (c0+1)%2."""
     d = info(name)
     p, keep = make_problem(name, n, steps, _np_dtype(t), coefs, divisor)
-    cfg = make_config(bt if bt else steps, variant, sl)
+    cfg = make_config(bt if bt else steps, variant, sl, mode)
     L = layout_of(t, d["dim"])
     _check(lib().an5d_launch(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()), ctypes.byref(L),
                              ctypes.byref(cfg), int(steps), int(c0), _stream_ptr(stream)))

@@ -52,11 +52,18 @@ typedef struct an5d_problem {
 
 /* Tiling.  bt: timesteps fused per launch (0: tuned/default).  variant:
  * index into an5d_variant_info() for this stencil x dtype (-1: tuned or
- * default).  sl: planes per streaming segment along dim 0 (0: default). */
+ * default).  sl: planes per streaming segment along dim 0 (0: default).
+ * mode: AN5D_EXACT (0) reproduces the C loop bit for bit (source term order,
+ * no contraction, correctly rounded division); AN5D_FAST (1) folds the
+ * divisor into the coefficients and chains FMAs (results within the
+ * tolerance checked by the tests, not bitwise). */
+#define AN5D_EXACT 0
+#define AN5D_FAST 1
 typedef struct an5d_config {
   int bt;
   int variant;
   int sl;
+  int mode;
 } an5d_config;
 
 /* Element strides of a device-resident problem: cell (i,j,k) of buffer b is

@@ -174,3 +174,25 @@ def test_rectangular_and_tiny_grids():
         pyan5d.forward_host(name, A, n, 7)
         O.run(name, B, n, 7)
         assert same(A[1], B[1])
+
+
+@pytest.mark.parametrize("name", ["j2d5pt", "j2d9pt", "star2d3r", "box2d2r", "gradient2d",
+                                  "star3d1r", "box3d1r", "j3d27pt", "star3d4r"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_fast_mode_within_tolerance(name, dt):
+    """FMA mode (divisor folded into the coefficients): within 1e-5 (FP32) /
+    1e-12 (FP64) relative of the oracle, relative to max(|ref|, smallest normal)."""
+    d = O.info(name)
+    n = [200, 700] if d["dim"] == 2 else [30, 60, 140]
+    T = 40 if d["dim"] == 2 else 12
+    A = rand_grid(name, n, dt, seed=21)
+    B = A.copy()
+    vs = pyan5d.variants(name, 0 if dt == np.float32 else 1)
+    bt = min(3, vs[0]["btmax"])
+    print(f"case fast {name} {np.dtype(dt).name} bt={bt}", flush=True)
+    pyan5d.forward_host(name, A, n, T, bt=bt, mode=pyan5d.FAST)
+    O.run(name, B, n, T)
+    got, ref = A[T % 2].astype(np.float64), B[T % 2].astype(np.float64)
+    floor = np.finfo(dt).tiny
+    rel = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor))
+    assert rel <= (1e-5 if dt == np.float32 else 1e-12), rel

@@ -13,11 +13,11 @@ using D = Def<SID>;
 // cells per lane and deepest block: keeps fr/ar rings under ~200 VGPRs
 constexpr int adj_ck() {
   if (!D::linear) return 2;
-  if (D::rad <= 2) return AN5D_F64 ? 2 : 4;
-  return 4;
+  if (D::rad == 1 && D::ntaps <= 5 && !AN5D_F64) return 4;
+  return D::rad <= 2 ? 2 : 4;  // a lane holds at least R cells
 }
 constexpr int adj_btmax() {
-  if (!D::linear) return AN5D_F64 ? 3 : 4;
+  if (!D::linear) return 2;
   if (D::rad == 1) return D::ntaps > 5 ? 3 : 4;
   if (D::rad == 2) return 2;
   return 1;
@@ -47,7 +47,11 @@ hipError_t launch_seed(const void* uT, const void* target, void* lam, double* lo
 }
 
 hipError_t launch_halo(const void* src, void* dst, int D0, int D2, long long s0, int R, hipStream_t st) {
-  hipLaunchKernelGGL((halo_copy_kernel<T>), dim3(1024), dim3(256), 0, st, static_cast<const T*>(src),
+  const long long cells = 2LL * R * D2 + 2LL * R * (D0 - 2 * R);
+  long long blocks = (cells + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((halo_copy_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const T*>(src),
                      static_cast<T*>(dst), D0, D2, s0, R);
   return hipGetLastError();
 }

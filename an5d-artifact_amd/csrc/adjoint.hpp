@@ -133,216 +133,220 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
     });
   }
 
-  // Rings start at zero and out-of-grid planes load as zero: every value the
-  // pipeline computes is then finite, so the masked products (mu == 0 off
-  // the interior) stay exactly zero, also for gradient2d's weights.
-  T fr[BT][LFMAX][CK];  // forward rings (stage m uses the last LF(m) slots)
-  T ar[BT][P][CK];      // adjoint rings (stage a output; stage 0 = lambda input)
+  // All rings share one length L = LFMAX and the main loop is unrolled by L,
+  // so rotating them is register renaming.  Forward stage m only reads its
+  // last LF(m) slots and adjoint stages their last P: the other slots are
+  // dead and cost nothing.  Rings start at zero and every load reads a real
+  // grid cell, so every value the pipeline computes is finite and the
+  // masked products (mu == 0 off the interior) stay exactly zero.
+  constexpr int L = LFMAX;
+  T fr[BT][L][CK];  // forward rings (stage 0 = checkpoint input)
+  T ar[BT][L][CK];  // adjoint rings (stage 0 = lambda input)
 #pragma unroll
   for (int m = 0; m < BT; m++)
 #pragma unroll
-    for (int p = 0; p < LFMAX; p++)
+    for (int p = 0; p < L; p++)
 #pragma unroll
-      for (int kk = 0; kk < CK; kk++) fr[m][p][kk] = T(0);
-#pragma unroll
-  for (int m = 0; m < BT; m++)
-#pragma unroll
-    for (int p = 0; p < P; p++)
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) ar[m][p][kk] = T(0);
+      for (int kk = 0; kk < CK; kk++) {
+        fr[m][p][kk] = T(0);
+        ar[m][p][kk] = T(0);
+      }
   double gsum[NG];
   T gpart[NG];
 #pragma unroll
   for (int g = 0; g < NG; g++) { gsum[g] = 0.0; gpart[g] = 0; }
-  int since_flush = 0;
 
+  // Out-of-grid planes and cells load a clamped (real, finite) neighbour:
+  // they only feed cells that are discarded or multiplied by mu == 0.
   auto load_row = [&](const T* base, int p, T (&dst)[CK]) {
-    if (p < 0 || p >= a.D0) {
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) dst[kk] = T(0);
-      return false;
-    }
-    const T* row = base + (long long)p * a.s0 + kl;
+    p = p < 0 ? 0 : (p >= a.D0 ? a.D0 - 1 : p);
+    const T* row = base + (long long)p * a.s0;
     if (kfull) {
-      load_run<T, CK>(dst, row);
+      load_run<T, CK>(dst, row + kl);
     } else {
 #pragma unroll
-      for (int kk = 0; kk < CK; kk++) dst[kk] = (kl + kk >= 0 && kl + kk < a.D2) ? row[kk] : T(0);
+      for (int kk = 0; kk < CK; kk++) {
+        int k = kl + kk;
+        k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+        dst[kk] = row[k];
+      }
     }
-    return true;
+  };
+  auto shift = [&](T (&ring)[L][CK], const T (&in)[CK]) {
+#pragma unroll
+    for (int p = 0; p < L - 1; p++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) ring[p][kk] = ring[p + 1][kk];
+#pragma unroll
+    for (int kk = 0; kk < CK; kk++) ring[L - 1][kk] = in[kk];
   };
 
-  for (int n = 0; n < nit; n++) {
-    const int pU = i0 - HU + n;
-    // ---- forward recompute ----------------------------------------------
+  // inputs are fetched one iteration ahead
+  T preU[CK], preL[CK];
+  load_row(a.U, i0 - HU, preU);
+  load_row(a.Lin, i0 - HU - DLT, preL);
+  const int nit_pad = (nit + L - 1) / L * L;
+  for (int n0 = 0; n0 < nit_pad; n0 += L) {
 #pragma unroll
-    for (int p = 0; p < LFMAX - 1; p++)
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) fr[0][p][kk] = fr[0][p + 1][kk];
-    load_row(a.U, pU, fr[0][LFMAX - 1]);
-    static_for<1, BT>([&](auto mc) {
-      constexpr int M = decltype(mc)::value;
-      constexpr int LP = LF(M - 1);     // input ring length
-      constexpr int B0 = LFMAX - LP;    // first used slot of the input ring
-      const int q = pU - M * R;
-      T xin[P][CK + 2 * XK];
-      static_for<0, P>([&](auto pc) {
-        constexpr int PP = decltype(pc)::value;
-        extend_row<T, CK, XK>(fr[M - 1][B0 + LP - P + PP], xin[PP]);
-      });
-      const bool pin = q >= R && q < a.D0 - R;
-      T out[CK];
-      static_for<0, CK>([&](auto kc) {
-        constexpr int KK = decltype(kc)::value;
-        RowGet<D, T, P, CK, XK, KK> get{xin, R};
-        const T val = eval_point<D, T>(get, c, div, has_div, gscale);
-        out[KK] = (pin && kin[KK]) ? val : xin[R][KK + XK];
-      });
-#pragma unroll
-      for (int p = LFMAX - LF(M); p < LFMAX - 1; p++)
-#pragma unroll
-        for (int kk = 0; kk < CK; kk++) fr[M][p][kk] = fr[M][p + 1][kk];
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) fr[M][LFMAX - 1][kk] = out[kk];
-    });
-    // ---- adjoint stages --------------------------------------------------
-#pragma unroll
-    for (int p = 0; p < P - 1; p++)
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) ar[0][p][kk] = ar[0][p + 1][kk];
-    load_row(a.Lin, pU - DLT, ar[0][P - 1]);
-    static_for<1, BT + 1>([&](auto ac) {
-      constexpr int A = decltype(ac)::value;
-      constexpr int M = BT - A;          // forward stage paired with this adjoint stage
-      constexpr int LM = LF(M);
-      constexpr int BM = LFMAX - LM;     // slot of plane qa - W in ring M
-      const int qa = pU - DLT - A * R;
-      const bool pin = qa >= R && qa < a.D0 - R;
-      const bool own = qa >= i0 && qa < i0 + sle;
-      // mu planes qa-R..qa+R, row-extended
-      T mx[P][CK + 2 * XK];
-      static_for<0, P>([&](auto pc) {
-        constexpr int PP = decltype(pc)::value;
-        extend_row<T, CK, XK>(ar[A - 1][PP], mx[PP]);
-      });
-      T out[CK];
-      if constexpr (D::linear) {
-#pragma clang fp contract(fast)
+    for (int u = 0; u < L; u++) {
+      const int n = n0 + u;
+      const int pU = i0 - HU + n;
+      shift(fr[0], preU);
+      shift(ar[0], preL);
+      load_row(a.U, pU + 1, preU);
+      load_row(a.Lin, pU + 1 - DLT, preL);
+      // ---- forward recompute (stages 1..BT-1) ------------------------------
+      static_for<1, BT>([&](auto mc) {
+        constexpr int M = decltype(mc)::value;
+        const int q = pU - M * R;
+        T xin[P][CK + 2 * XK];
+        static_for<0, P>([&](auto pc) {
+          constexpr int PP = decltype(pc)::value;
+          extend_row<T, CK, XK>(fr[M - 1][L - P + PP], xin[PP]);
+        });
+        const bool pin = q >= R && q < a.D0 - R;
+        T out[CK];
         static_for<0, CK>([&](auto kc) {
           constexpr int KK = decltype(kc)::value;
-          T acc = T(0);
-          static_for<0, NT>([&](auto qc) {
-            constexpr int Q = decltype(qc)::value;
-            constexpr TapDef tp = D::taps[Q];
-            acc = acc + ct[Q] * mx[R - tp.d0][KK - tp.d2 + XK];
-          });
-          out[KK] = (pin && kin[KK]) ? acc : T(0);
-        });
-        if (own) {
-          // sum_x mu(x) u_t(x + d_q) over owned cells of plane qa
-          T ux[2 * R + 1][CK + 2 * XK];
-          static_for<0, 2 * R + 1>([&](auto pc) {
-            constexpr int PP = decltype(pc)::value;
-            extend_row<T, CK, XK>(fr[M][BM + (W - R) + PP], ux[PP]);
-          });
-          static_for<0, CK>([&](auto kc) {
-            constexpr int KK = decltype(kc)::value;
-            const T mu = kst[KK] ? mx[R][KK + XK] : T(0);
+          RowGet<D, T, P, CK, XK, KK> get{xin, R};
+          T val;
+          if constexpr (D::linear) {
+            // folded coefficients s_q c_q / div, FMA chain
+#pragma clang fp contract(fast)
+            val = T(0);
             static_for<0, NT>([&](auto qc) {
               constexpr int Q = decltype(qc)::value;
               constexpr TapDef tp = D::taps[Q];
-              gpart[Q] = gpart[Q] + mu * ux[R + tp.d0][KK + tp.d2 + XK];
+              val = fma_t(ct[Q], xin[R + tp.d0][KK + tp.d2 + XK], val);
             });
+          } else {
+            val = eval_point<D, T>(get, c, div, has_div, gscale);
+          }
+          out[KK] = (pin && kin[KK]) ? val : xin[R][KK + XK];
+        });
+        shift(fr[M], out);
+      });
+      // ---- adjoint stages ---------------------------------------------------
+      static_for<1, BT + 1>([&](auto ac) {
+        constexpr int A = decltype(ac)::value;
+        constexpr int M = BT - A;          // forward stage paired with this adjoint stage
+        constexpr int BM = L - LF(M);      // slot of plane qa - W in ring M
+        const int qa = pU - DLT - A * R;
+        const bool pin = qa >= R && qa < a.D0 - R;
+        const bool own = qa >= i0 && qa < i0 + sle && n < nit;
+        // mu planes qa-R..qa+R, row-extended
+        T mx[P][CK + 2 * XK];
+        static_for<0, P>([&](auto pc) {
+          constexpr int PP = decltype(pc)::value;
+          extend_row<T, CK, XK>(ar[A - 1][L - P + PP], mx[PP]);
+        });
+        T out[CK];
+        if constexpr (D::linear) {
+#pragma clang fp contract(fast)
+          static_for<0, CK>([&](auto kc) {
+            constexpr int KK = decltype(kc)::value;
+            T acc = T(0);
+            static_for<0, NT>([&](auto qc) {
+              constexpr int Q = decltype(qc)::value;
+              constexpr TapDef tp = D::taps[Q];
+              acc = fma_t(ct[Q], mx[R - tp.d0][KK - tp.d2 + XK], acc);
+            });
+            out[KK] = (pin && kin[KK]) ? acc : T(0);
+          });
+          if (own) {
+            // sum_x mu(x) u_t(x + d_q) over owned cells of plane qa
+            T ux[2 * R + 1][CK + 2 * XK];
+            static_for<0, 2 * R + 1>([&](auto pc) {
+              constexpr int PP = decltype(pc)::value;
+              extend_row<T, CK, XK>(fr[M][BM + (W - R) + PP], ux[PP]);
+            });
+            static_for<0, CK>([&](auto kc) {
+              constexpr int KK = decltype(kc)::value;
+              const T mu = kst[KK] ? mx[R][KK + XK] : T(0);
+              static_for<0, NT>([&](auto qc) {
+                constexpr int Q = decltype(qc)::value;
+                constexpr TapDef tp = D::taps[Q];
+                gpart[Q] = fma_t(mu, ux[R + tp.d0][KK + tp.d2 + XK], gpart[Q]);
+              });
+            });
+          }
+        } else {
+          // gradient2d: F(x) = C + scale * rs(x), rs = (eps + sum_m (C - N_m)^2)^-1/2
+          //   dF/dC = 1 + f * sum_m (C - N_m), dF/dN_m = -f (C - N_m), f = -scale rs^3
+          // lambda(y) = mu(y) dF/dC(y) + sum_m mu(y - e_m) dF/dN_m(y - e_m)
+#pragma clang fp contract(fast)
+          T ux[5][CK + 2 * XU];
+          static_for<0, 5>([&](auto pc) {
+            constexpr int PP = decltype(pc)::value;
+            extend_row<T, CK, XU>(fr[M][BM + PP], ux[PP]);
+          });
+          const T eps = c[0];
+          const T sc = (T)gscale;
+          T fN[CK], fS[CK], gE[CK], gW[CK], gC[CK];
+          static_for<0, CK>([&](auto kc) {
+            constexpr int KK = decltype(kc)::value;
+            constexpr int K = KK + XU;
+            {  // x on plane qa+1: its N neighbour is y
+              const T cc = ux[3][K];
+              const T dn = cc - ux[2][K], ds = cc - ux[4][K], de = cc - ux[3][K + 1], dw = cc - ux[3][K - 1];
+              const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
+              const T rs = T(1) / __builtin_sqrt(s2);
+              const T f = -sc * rs * rs * rs;
+              fN[KK] = mx[R + 1][KK + XK] * (-f * dn);
+            }
+            {  // x on plane qa-1: its S neighbour is y
+              const T cc = ux[1][K];
+              const T dn = cc - ux[0][K], ds = cc - ux[2][K], de = cc - ux[1][K + 1], dw = cc - ux[1][K - 1];
+              const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
+              const T rs = T(1) / __builtin_sqrt(s2);
+              const T f = -sc * rs * rs * rs;
+              fS[KK] = mx[R - 1][KK + XK] * (-f * ds);
+            }
+            {  // x on plane qa: own centre term and the E/W fluxes
+              const T cc = ux[2][K];
+              const T dn = cc - ux[1][K], ds = cc - ux[3][K], de = cc - ux[2][K + 1], dw = cc - ux[2][K - 1];
+              const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
+              const T rs = T(1) / __builtin_sqrt(s2);
+              const T f = -sc * rs * rs * rs;
+              const T mu = mx[R][KK + XK];
+              gC[KK] = mu * (T(1) + f * (dn + ds + de + dw));
+              gE[KK] = mu * (-f * de);  // to x + (0,1)
+              gW[KK] = mu * (-f * dw);  // to x - (0,1)
+              if (own) {
+                const T muo = kst[KK] ? mu : T(0);
+                gpart[0] = gpart[0] + muo * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
+                gpart[1] = gpart[1] + muo * rs;           // d/d scale
+              }
+            }
+          });
+          T gEx[CK + 2], gWx[CK + 2];
+          extend_row<T, CK, 1>(gE, gEx);
+          extend_row<T, CK, 1>(gW, gWx);
+          static_for<0, CK>([&](auto kc) {
+            constexpr int KK = decltype(kc)::value;
+            const T v = gC[KK] + fN[KK] + fS[KK] + gEx[KK] + gWx[KK + 2];
+            out[KK] = (pin && kin[KK]) ? v : T(0);
           });
         }
-      } else {
-        // gradient2d: F(x) = C + scale * rs(x), rs = (eps + sum_m (C - N_m)^2)^-1/2
-        //   dF/dC = 1 + f * sum_m (C - N_m), dF/dN_m = -f (C - N_m), f = -scale rs^3
-        // lambda(y) = mu(y) dF/dC(y) + sum_m mu(y - e_m) dF/dN_m(y - e_m)
-#pragma clang fp contract(fast)
-        T ux[5][CK + 2 * XU];
-        static_for<0, 5>([&](auto pc) {
-          constexpr int PP = decltype(pc)::value;
-          extend_row<T, CK, XU>(fr[M][BM + PP], ux[PP]);
-        });
-        const T eps = c[0];
-        const T sc = (T)gscale;
-        // per plane qa-1, qa, qa+1 (index 1..3 in ux): f and the four differences
-        T fN[CK], fS[CK], gE[CK], gW[CK], gC[CK];
-        static_for<0, CK>([&](auto kc) {
-          constexpr int KK = decltype(kc)::value;
-          constexpr int K = KK + XU;
-          // x on plane qa+1 (ux row 3): contributes dF/dN (its N neighbour is y)
-          {
-            const T cc = ux[3][K];
-            const T dn = cc - ux[2][K], ds = cc - ux[4][K], de = cc - ux[3][K + 1], dw = cc - ux[3][K - 1];
-            const T s = eps + dn * dn + ds * ds + de * de + dw * dw;
-            const T rs = T(1) / __builtin_sqrt(s);
-            const T f = -sc * rs * rs * rs;
-            fN[KK] = mx[R + 1][KK + XK] * (-f * dn);
-          }
-          // x on plane qa-1 (ux row 1): its S neighbour is y
-          {
-            const T cc = ux[1][K];
-            const T dn = cc - ux[0][K], ds = cc - ux[2][K], de = cc - ux[1][K + 1], dw = cc - ux[1][K - 1];
-            const T s = eps + dn * dn + ds * ds + de * de + dw * dw;
-            const T rs = T(1) / __builtin_sqrt(s);
-            const T f = -sc * rs * rs * rs;
-            fS[KK] = mx[R - 1][KK + XK] * (-f * ds);
-          }
-          // x on plane qa: own centre term and the E/W fluxes to the neighbours
-          {
-            const T cc = ux[2][K];
-            const T dn = cc - ux[1][K], ds = cc - ux[3][K], de = cc - ux[2][K + 1], dw = cc - ux[2][K - 1];
-            const T s = eps + dn * dn + ds * ds + de * de + dw * dw;
-            const T rs = T(1) / __builtin_sqrt(s);
-            const T f = -sc * rs * rs * rs;
-            const T mu = mx[R][KK + XK];
-            gC[KK] = mu * (T(1) + f * (dn + ds + de + dw));
-            gE[KK] = mu * (-f * de);  // to x + (0,1)
-            gW[KK] = mu * (-f * dw);  // to x - (0,1)
-            if (own) {
-              const T muo = kst[KK] ? mu : T(0);
-              gpart[0] = gpart[0] + muo * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
-              gpart[1] = gpart[1] + muo * rs;                  // d/d scale
+        if constexpr (A < BT) {
+          shift(ar[A], out);
+        } else {
+          if (own) {
+            T* row = a.Lout + (long long)qa * a.s0 + kl;
+            if (kst_all) {
+              store_run<T, CK>(row, out);
+            } else {
+#pragma unroll
+              for (int kk = 0; kk < CK; kk++)
+                if (kst[kk]) row[kk] = out[kk];
             }
           }
-        });
-        // y gets gE from x = y - (0,1) and gW from x = y + (0,1)
-        T gEx[CK + 2], gWx[CK + 2];
-        extend_row<T, CK, 1>(gE, gEx);
-        extend_row<T, CK, 1>(gW, gWx);
-        static_for<0, CK>([&](auto kc) {
-          constexpr int KK = decltype(kc)::value;
-          const T v = gC[KK] + fN[KK] + fS[KK] + gEx[KK] + gWx[KK + 2];
-          out[KK] = (pin && kin[KK]) ? v : T(0);
-        });
-      }
-      if constexpr (A < BT) {
-#pragma unroll
-        for (int p = 0; p < P - 1; p++)
-#pragma unroll
-          for (int kk = 0; kk < CK; kk++) ar[A][p][kk] = ar[A][p + 1][kk];
-#pragma unroll
-        for (int kk = 0; kk < CK; kk++) ar[A][P - 1][kk] = out[kk];
-      } else {
-        if (own) {
-          T* row = a.Lout + (long long)qa * a.s0 + kl;
-          if (kst_all) {
-            store_run<T, CK>(row, out);
-          } else {
-#pragma unroll
-            for (int kk = 0; kk < CK; kk++)
-              if (kst[kk]) row[kk] = out[kk];
-          }
         }
-      }
-    });
-    if (++since_flush == 16) {
-#pragma unroll
-      for (int g = 0; g < NG; g++) { gsum[g] += (double)gpart[g]; gpart[g] = 0; }
-      since_flush = 0;
+      });
     }
+    // flush the per-lane partial sums to FP64 once per unrolled body
+#pragma unroll
+    for (int g = 0; g < NG; g++) { gsum[g] += (double)gpart[g]; gpart[g] = 0; }
   }
   static_for<0, NG>([&](auto gc) {
     constexpr int G = decltype(gc)::value;
@@ -376,15 +380,28 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(const T* __restrict__
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, part);
 }
 
-// Copy the halo cells of `src` into `dst` (same layout), for the checkpoint slots.
+// Copy the halo cells of `src` into `dst` (same layout), for the checkpoint
+// slots: R full rows at the top and bottom, R cells at both ends of the
+// other rows.  Only halo cells are visited.
 template <typename T>
 __global__ __launch_bounds__(256) void halo_copy_kernel(const T* __restrict__ src, T* __restrict__ dst,
                                                         int D0, int D2, long long s0, int R) {
-  const long long total = (long long)D0 * D2;
+  const long long rows_tb = 2LL * R * D2;               // top and bottom rows
+  const long long sides = 2LL * R * (long long)(D0 - 2 * R);  // left/right cells of the other rows
+  const long long total = rows_tb + sides;
   for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx / D2), k = (int)(idx % D2);
-    if (i >= R && i < D0 - R && k >= R && k < D2 - R) continue;
+    int i, k;
+    if (idx < rows_tb) {
+      const int r = (int)(idx / D2);
+      i = r < R ? r : D0 - 2 * R + r;
+      k = (int)(idx % D2);
+    } else {
+      const long long j = idx - rows_tb;
+      i = R + (int)(j / (2 * R));
+      const int c = (int)(j % (2 * R));
+      k = c < R ? c : D2 - 2 * R + c;
+    }
     const long long x = (long long)i * s0 + k;
     dst[x] = src[x];
   }

@@ -280,12 +280,15 @@ int default_bt(const StencilRT& s, int dtype) {
   return s.rad == 1 ? 2 : 1;
 }
 
-int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout, int* btout) {
+int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout, int* btout,
+            int* slout) {
   auto vs = variants_of(p->stencil, p->dtype);
   if (vs.empty()) return fail(AN5D_E_UNSUPPORTED, "no kernels compiled for this stencil/dtype");
   const StencilRT& s = stencil_rt(p->stencil);
   int want_bt = cfg ? cfg->bt : 0;
   int want_v = cfg ? cfg->variant : -1;
+  int sl = cfg ? cfg->sl : 0;
+  const int mode = cfg ? cfg->mode : AN5D_EXACT;
   if (want_v >= (int)vs.size()) return fail(AN5D_E_INVALID, "variant index out of range");
   if (want_bt < 0 || want_bt > kMaxBT) return fail(AN5D_E_INVALID, "bt out of range");
   const Variant* v = nullptr;
@@ -293,20 +296,23 @@ int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout,
     v = vs[want_v];
   } else {
     TunedChoice tc;
-    if (tuned_lookup(p, &tc) && tc.variant < (int)vs.size()) {
+    if (tuned_lookup(p, mode, &tc) && tc.variant < (int)vs.size() &&
+        (want_bt == 0 || want_bt <= vs[tc.variant]->btmax)) {
       v = vs[tc.variant];
       if (want_bt == 0) want_bt = tc.bt;
+      if (sl == 0) sl = tc.sl;
     } else {
       v = vs[0];  // first sweep shape, or the direct kernel when none exists
     }
   }
   int bt = want_bt ? want_bt : default_bt(s, p->dtype);
   if (bt > v->btmax) {
-    if (want_bt) return fail(AN5D_E_UNSUPPORTED, "bt=" + std::to_string(bt) + " exceeds the variant's btmax=" + std::to_string(v->btmax));
+    if (want_bt && cfg && cfg->bt) return fail(AN5D_E_UNSUPPORTED, "bt=" + std::to_string(bt) + " exceeds the variant's btmax=" + std::to_string(v->btmax));
     bt = v->btmax;
   }
   *vout = v;
   *btout = bt;
+  *slout = sl;
   return AN5D_OK;
 }
 
@@ -411,8 +417,8 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
       L.buf_stride < g.D0 * L.s0)
     return fail(AN5D_E_INVALID, "layout strides smaller than the grid");
   const Variant* v = nullptr;
-  int bt = 0;
-  int rc = resolve(p, cfg, &v, &bt);
+  int bt = 0, sl = 0;
+  int rc = resolve(p, cfg, &v, &bt, &sl);
   if (rc) return rc;
   std::vector<unsigned char> cbytes;
   int has_div = 0;
@@ -433,7 +439,6 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
     steps.resize(n);
     an5d_schedule(p->timesteps, v->kind == kDirect ? 1 : bt, steps.data(), n);
   }
-  const int sl = cfg ? cfg->sl : 0;
   int c0 = c0_first;
   for (int k : steps) {
     LaunchPlan lp;

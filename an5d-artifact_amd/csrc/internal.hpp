@@ -50,7 +50,8 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
                  const an5d_config* cfg, int steps_only, int c0_first, hipStream_t st);
 double sb_time_ms();
 
-// autotuner cache (autotune.cpp)
-bool tuned_lookup(const an5d_problem* p, TunedChoice* out);
+// autotuner table (autotune.cpp)
+bool tuned_lookup(const an5d_problem* p, int mode, TunedChoice* out);
+double model_gstencil(const an5d_problem* p, const Variant& v, int bt, int sl, int mode);
 
 }  // namespace an5d

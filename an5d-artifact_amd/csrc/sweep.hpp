@@ -48,12 +48,14 @@ struct SweepArgs {
 };
 
 // ---- cross-lane moves ------------------------------------------------------
-// lane L receives lane L-1's value (wave_shr:1) / lane L+1's value (wave_shl:1)
+// lane L receives lane L-1's value (wave_shr:1) / lane L+1's value
+// (wave_shl:1).  bound_ctrl: the lane without a source reads 0, so no "old"
+// operand has to be materialised first.
 __device__ __forceinline__ int dpp_shr1(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
 }
 __device__ __forceinline__ int dpp_shl1(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);
 }
 __device__ __forceinline__ float from_left(float v) {
   return __builtin_bit_cast(float, dpp_shr1(__builtin_bit_cast(int, v)));
@@ -232,8 +234,10 @@ constexpr size_t sweep_lds_bytes() {
     return 0;
 }
 
+// tile >= 0 (2D only): process that wave tile instead of the one implied by
+// blockIdx (used by the batched work-queue kernel).
 template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT, int DM, int UNR>
-__device__ __forceinline__ void sweep_body(const SweepArgs<T>& a) {
+__device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1) {
   using D = Def<ID>;
   constexpr int R = D::rad;
   constexpr int P = 2 * R + 1;
@@ -261,7 +265,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a) {
     tj = b % a.ntj;
     seg = b / a.ntj;
   } else {
-    const int gw = blockIdx.x * NW + w;
+    const int gw = tile >= 0 ? tile : blockIdx.x * NW + w;
     if (gw >= a.ntk * a.nseg) return;
     tk = gw % a.ntk;
     seg = gw / a.ntk;
@@ -441,32 +445,34 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a) {
     }
   };
 
-  // the next input plane is loaded one iteration ahead of its use
+  // The next input plane is loaded one iteration ahead of its use.  The
+  // iteration count is padded to a multiple of UNR so the unrolled body has
+  // no exit inside it: the ring rotation then costs no register moves.  The
+  // padding iterations only produce planes past the segment, never stored.
   Plane<T, CJ, CK> pre;
   if (i0 - H >= 0) load_plane(pre, i0 - H);
-  for (int n0 = 0; n0 < nit; n0 += UNR) {
+  const int nit_pad = (nit + UNR - 1) / UNR * UNR;
+  for (int n0 = 0; n0 < nit_pad; n0 += UNR) {
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
       const int n = n0 + u;
-      if (n < nit) {
-        const int p_in = i0 - H + n;
+      const int p_in = i0 - H + n;
 #pragma unroll
-        for (int p = 0; p < P - 1; p++) ring[0][p] = ring[0][p + 1];
-        ring[0][P - 1] = pre;
-        if (n + 1 < nit && p_in + 1 >= 0 && p_in + 1 < a.D0) load_plane(pre, p_in + 1);
-        static_for<1, BT + 1>([&](auto sc) {
-          constexpr int S = decltype(sc)::value;
-          Plane<T, CJ, CK> out;
-          compute(ring[S - 1], out, p_in - S * R);
-          if constexpr (S < BT) {
+      for (int p = 0; p < P - 1; p++) ring[0][p] = ring[0][p + 1];
+      ring[0][P - 1] = pre;
+      if (n + 1 < nit && p_in + 1 >= 0 && p_in + 1 < a.D0) load_plane(pre, p_in + 1);
+      static_for<1, BT + 1>([&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        Plane<T, CJ, CK> out;
+        compute(ring[S - 1], out, p_in - S * R);
+        if constexpr (S < BT) {
 #pragma unroll
-            for (int p = 0; p < P - 1; p++) ring[S][p] = ring[S][p + 1];
-            ring[S][P - 1] = out;
-          } else {
-            if (n >= 2 * BT * R) store_plane(out, p_in - BT * R);
-          }
-        });
-      }
+          for (int p = 0; p < P - 1; p++) ring[S][p] = ring[S][p + 1];
+          ring[S][P - 1] = out;
+        } else {
+          if (n >= 2 * BT * R && n < nit) store_plane(out, p_in - BT * R);
+        }
+      });
     }
   }
 }

@@ -86,6 +86,16 @@ def lib():
         L.an5d_adjoint_device.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                           P(Config), ctypes.c_void_p, P(ctypes.c_double)]
+        L.an5d_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Problem), P(ctypes.c_void_p),
+                                        P(Config)]
+        L.an5d_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.an5d_batch_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int), P(ctypes.c_longlong)]
+        L.an5d_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.an5d_autotune.argtypes = [P(Problem), ctypes.c_void_p, P(Layout), ctypes.c_int, ctypes.c_int,
+                                    P(Config), P(ctypes.c_double)]
+        L.an5d_model_estimate.argtypes = [P(Problem), P(Config), P(ctypes.c_double)]
+        L.an5d_tune_load.argtypes = [ctypes.c_char_p]
+        L.an5d_tune_save.argtypes = [ctypes.c_char_p]
         L.an5d_cpu_forward.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double)]
         _lib = L
     return _lib
@@ -302,6 +312,7 @@ class AdjointWorkspace:
 
 def adjoint_device(name, u0, target, grad, gcoef, loss, n, T, ws, coefs=None, divisor=-1.0,
                    bt=0, variant=-1, sl=0, stream=None, time_forward=False):
+    # ws must have been sized for the same bt
     """Device tensors: u0/target/grad single grids, gcoef/loss float64 tensors."""
     p, keep = make_problem(name, n, T, _np_dtype(u0), coefs, divisor)
     cfg = make_config(bt, variant, sl)
@@ -313,3 +324,76 @@ def adjoint_device(name, u0, target, grad, gcoef, loss, n, T, ws, coefs=None, di
                                      _stream_ptr(stream), ctypes.byref(mf) if time_forward else None))
     del keep
     return mf.value if time_forward else None
+
+
+class Batch:
+    """Batched forward over many device grids (an5d_batch_*).  `items` is a
+    list of (name, n, T, tensor[2, ...], coefs, divisor); optional per-item bt."""
+
+    def __init__(self, items, bt=0, sl=0):
+        n = len(items)
+        probs = (Problem * n)()
+        bufs = (ctypes.c_void_p * n)()
+        cfgs = (Config * n)()
+        self._keep = []
+        for i, (name, ext, T, t, coefs, divisor) in enumerate(items):
+            p, keep = make_problem(name, ext, T, _np_dtype(t), coefs, divisor)
+            probs[i] = p
+            self._keep.append(keep)
+            bufs[i] = t.data_ptr()
+            cfgs[i] = make_config(bt, -1, sl)
+        h = ctypes.c_void_p()
+        _check(lib().an5d_batch_create(ctypes.byref(h), n, probs, bufs, cfgs))
+        self.handle = h
+        lv, un = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().an5d_batch_info(h, ctypes.byref(lv), ctypes.byref(un)))
+        self.levels, self.units = lv.value, un.value
+
+    def run(self, stream=None):
+        _check(lib().an5d_batch_run(self.handle, _stream_ptr(stream)))
+
+    def close(self):
+        if self.handle:
+            lib().an5d_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None):
+    """Rank configurations with the model, time the best on the GPU, record
+    the winner.  Returns (config dict, GStencil/s)."""
+    p, keep = make_problem(name, n, 1, dtype)
+    best = Config()
+    gs = ctypes.c_double()
+    ptr, lay = None, None
+    if tensor is not None:
+        ptr = ctypes.c_void_p(tensor.data_ptr())
+        lay = ctypes.byref(layout_of(tensor, info(name)["dim"]))
+    _check(lib().an5d_autotune(ctypes.byref(p), ptr, lay, int(mode), int(max_measured),
+                               ctypes.byref(best), ctypes.byref(gs)))
+    return dict(bt=best.bt, variant=best.variant, sl=best.sl, mode=best.mode), gs.value
+
+
+def model_estimate(name, n, dtype=np.float32, bt=1, variant=0, sl=0, mode=EXACT):
+    p, keep = make_problem(name, n, 1, dtype)
+    cfg = make_config(bt, variant, sl, mode)
+    gs = ctypes.c_double()
+    _check(lib().an5d_model_estimate(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(gs)))
+    return gs.value
+
+
+def tune_save(path=None):
+    _check(lib().an5d_tune_save(path.encode() if path else None))
+
+
+def tune_load(path=None):
+    _check(lib().an5d_tune_load(path.encode() if path else None))
+
+
+def tune_clear():
+    _check(lib().an5d_tune_clear())

@@ -41,3 +41,23 @@ def coefficients(name, preset="baseline"):
     if info(name)["linear"]:
         return seeded_normalised(name), 0.0
     return None, -1.0
+
+
+def batch_workload(count=96, seed=2024):
+    """The batched configuration: `count` instances, stencil uniform over
+    {j2d5pt, j2d9pt, star2d2r}, side log-uniform in [256, 4096], half square
+    and half 1:4 rectangles, T uniform in [64, 1024].  Returns a list of
+    (stencil, [n0, n1], T)."""
+    rng = np.random.default_rng(seed)
+    kinds = ["j2d5pt", "j2d9pt", "star2d2r"]
+    out = []
+    for i in range(count):
+        name = kinds[int(rng.integers(0, 3))]
+        side = int(round(float(np.exp(rng.uniform(np.log(256), np.log(4096))))))
+        if rng.random() < 0.5:
+            n = [side, side]
+        else:
+            n = [max(1, side // 4), side] if rng.random() < 0.5 else [side, max(1, side // 4)]
+        T = int(rng.integers(64, 1025))
+        out.append((name, n, T))
+    return out

@@ -81,6 +81,112 @@ def cpu_baseline(wl, budget_s=12.0):
             "kind": "port", "sample": f"oracle sequential loop, {'x'.join(map(str, n))}, {steps} steps"}
 
 
+def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3):
+    """(checkpointed forward + adjoint) / plain forward, 2D, FP32."""
+    n = list(n)
+    d = pyan5d.info(name)
+    coefs, div = presets.coefficients(name, "baseline")
+    g = torch.Generator(device="cuda").manual_seed(7)
+    shape = [x + 2 * d["rad"] for x in n]
+    u0 = torch.rand(shape, device="cuda", generator=g)
+    target = torch.rand(shape, device="cuda", generator=g)
+    grid = torch.stack([u0, u0])
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    # plain forward: best of the exact-mode bt values
+    best = None
+    for bt in (2, 3, 4, 6):
+        if bt > pyan5d.variants(name)[0]["btmax"]:
+            continue
+        pyan5d.forward_device(name, grid, n, T, coefs=coefs, divisor=div, bt=bt)
+        torch.cuda.synchronize()
+        a, b = ev(), ev()
+        a.record()
+        for _ in range(reps):
+            pyan5d.forward_device(name, grid, n, T, coefs=coefs, divisor=div, bt=bt)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        if best is None or ms < best[0]:
+            best = (ms, bt)
+    grad = torch.zeros_like(u0)
+    gc = torch.zeros(d["ntaps"] if d["linear"] else 2, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    best_fa = None
+    for abt in (2, 3, 4):
+        if abt > 1 and not d["linear"] and abt > 2:
+            continue
+        for sl in (0, 128, 256):
+            try:
+                ws = pyan5d.AdjointWorkspace(name, n, T, np.float32, bt=abt, sl=sl)
+            except pyan5d.An5dError:
+                continue
+            pyan5d.adjoint_device(name, u0, target, grad, gc, loss, n, T, ws, coefs=coefs, divisor=div,
+                                  bt=abt, sl=sl)
+            torch.cuda.synchronize()
+            a, b = ev(), ev()
+            a.record()
+            for _ in range(reps):
+                pyan5d.adjoint_device(name, u0, target, grad, gc, loss, n, T, ws, coefs=coefs, divisor=div,
+                                      bt=abt, sl=sl)
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / reps
+            log(f"[bench]   {name} fwd+adj bt={abt} sl={sl}: {ms:.3f} ms")
+            if best_fa is None or ms < best_fa[0]:
+                best_fa = (ms, abt, sl)
+            del ws
+            torch.cuda.empty_cache()
+    cells = float(np.prod(n))
+    return {"stencil": name, "n": n, "timesteps": T, "forward_ms": round(best[0], 3), "forward_bt": best[1],
+            "forward_gstencil": round(T * cells / best[0] / 1e6, 1), "fwd_adj_ms": round(best_fa[0], 3),
+            "fwd_adj_bt": best_fa[1], "fwd_adj_sl": best_fa[2], "ratio": round(best_fa[0] / best[0], 3)}
+
+
+def bench_batched(torch, pyan5d, presets, reps=2):
+    """96 instances: plain per-grid launch sequences vs the batched graph."""
+    work = presets.batch_workload()
+    g = torch.Generator(device="cuda").manual_seed(11)
+    items = []
+    total = 0.0
+    for name, n, T in work:
+        d = pyan5d.info(name)
+        u = torch.rand([x + 2 * d["rad"] for x in n], device="cuda", generator=g)
+        coefs, div = presets.coefficients(name, "baseline")
+        items.append((name, n, T, torch.stack([u, u]), coefs, div))
+        total += T * float(np.prod(n))
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def plain():
+        for name, n, T, t, coefs, div in items:
+            pyan5d.forward_device(name, t, n, T, coefs=coefs, divisor=div, bt=4)
+
+    plain()
+    torch.cuda.synchronize()
+    a, b = ev(), ev()
+    a.record()
+    for _ in range(reps):
+        plain()
+    b.record()
+    torch.cuda.synchronize()
+    ms_plain = a.elapsed_time(b) / reps
+    batch = pyan5d.Batch(items)
+    batch.run()
+    torch.cuda.synchronize()
+    a, b = ev(), ev()
+    a.record()
+    for _ in range(reps):
+        batch.run()
+    b.record()
+    torch.cuda.synchronize()
+    ms_batch = a.elapsed_time(b) / reps
+    levels = batch.levels
+    batch.close()
+    return {"instances": len(items), "updates": total, "levels": levels,
+            "plain_ms": round(ms_plain, 3), "plain_gstencil": round(total / ms_plain / 1e6, 1),
+            "batched_ms": round(ms_batch, 3), "batched_gstencil": round(total / ms_batch / 1e6, 1),
+            "speedup": round(ms_plain / ms_batch, 3)}
+
+
 def parse_profile_traffic(kernel_tag):
     """HBM bytes per launch of the dominant kernel from a committed rocprofv3
     counter summary (profiles/*traffic*.json written from the --pmc runs)."""
@@ -103,6 +209,11 @@ def main():
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--sl", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the fwd+adj and batched legs")
+    ap.add_argument("--mode", type=int, default=0, help="0 exact (bitwise), 1 fast (FMA, tolerance)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain 1 GiB device-to-device copy (scale reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -127,6 +238,16 @@ def main():
     bt = args.bt if args.bt > 0 else (2 if d["dim"] == 3 else 4)
     bt = min(bt, variants[variant]["btmax"])
 
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # 1 GiB read + 1 GiB written
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration copy of 1 GiB done")
+
     log(f"[bench] workload {wl['desc']}; variant {variants[variant]} bt={bt}")
     host = pyan5d.init_grid(name, n, dt, mode=1)
     grid = torch.from_numpy(host).cuda()
@@ -142,12 +263,12 @@ def main():
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
                 pyan5d.launch(name, grid, n, k, c0=c0, coefs=coefs, divisor=divisor,
-                              variant=variant, sl=args.sl, stream=stream)
+                              variant=variant, sl=args.sl, stream=stream, mode=args.mode)
                 e1.record(stream)
                 events.append((e0, e1))
             else:
                 pyan5d.launch(name, grid, n, k, c0=c0, coefs=coefs, divisor=divisor,
-                              variant=variant, sl=args.sl, stream=stream)
+                              variant=variant, sl=args.sl, stream=stream, mode=args.mode)
             c0 += 1
 
     for w in range(args.warmup):
@@ -176,6 +297,19 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": parse_profile_traffic(tag)}
 
+    del grid
+    torch.cuda.empty_cache()
+    extra = {}
+    if not args.no_extra:
+        log("[bench] forward+adjoint legs ...")
+        extra["fwd_adj"] = [bench_fwd_adj(torch, pyan5d, presets, nm) for nm in ("gradient2d", "j2d9pt", "j2d9pt-gol")]
+        log(f"[bench] {extra['fwd_adj']}")
+        torch.cuda.empty_cache()
+        log("[bench] batched leg ...")
+        extra["batched"] = bench_batched(torch, pyan5d, presets)
+        log(f"[bench] {extra['batched']}")
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -189,6 +323,7 @@ def main():
         "dtype": "f32" if dt == np.float32 else "f64",
         "data": "synthetic: uniform [0,1) from rand_r seeded with cell index + N; halo fixed",
         "config": {"workload": wl["desc"], "stencil": name, "n": n, "timesteps": T, "bt": bt,
+                   "mode": "fast (FMA, tolerance-checked)" if args.mode else "exact (bitwise vs the C loop)",
                    "variant": variants[variant], "sl": args.sl,
                    "gflops": round(gstencil * d["fpp"], 2),
                    "bt_scaled_hbm_roof_gstencil": HBM_PEAK_GBS * bt / (2 * esz),
@@ -196,6 +331,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -166,6 +166,38 @@ int an5d_adjoint_host(const an5d_problem* prob, const void* u0, const void* targ
                       double* gcoef, double* loss, const an5d_config* cfg, double* ms_fwd,
                       double* ms_total);
 
+/* ---- batched forward (2D, FP32) ------------------------------------------
+ * No counterpart in the reference (one grid per binary).  Advances n
+ * independent grids (stencils j2d5pt, j2d9pt, star2d2r, j2d9pt-gol; any sizes
+ * and timesteps) together: launch l of every instance's schedule goes into
+ * one kernel whose waves pull (instance, tile) work units from a device
+ * queue; the level sequence is captured into a HIP graph on the first run.
+ * dA[i]: both buffers of instance i, reference layout.  After a run, buffer
+ * T_i%2 of instance i holds its u_T, bit for bit the single-grid result.
+ * cfgs (optional): per-instance bt (<= 4) and sl. */
+typedef struct an5d_batch an5d_batch;
+int an5d_batch_create(an5d_batch** out, int n, const an5d_problem* probs, void* const* dA,
+                      const an5d_config* cfgs);
+int an5d_batch_run(an5d_batch* batch, void* stream);
+int an5d_batch_info(const an5d_batch* batch, int* levels, long long* work_units);
+int an5d_batch_destroy(an5d_batch* batch);
+
+/* ---- autotuner ------------------------------------------------------------
+ * Replaces the reference's Excel performance model + top-5 compile/run
+ * scripts (Model_V100.xlsm, compiled/nvcc_compile_top5.sh, compiled/run.sh).
+ * an5d_autotune ranks every (kernel shape, bt, sl) for `prob` with the
+ * analytic model (an5d_model_estimate), times the best max_measured of them
+ * on the GPU (dA: the caller's device grid, or NULL for a scratch grid) and
+ * records the winner in the table that the forward entry points consult
+ * when cfg->variant < 0.  an5d_tune_save/load persist the table as text
+ * (default file: an5d-artifact_amd/tuned.tsv, or $AN5D_TUNE_FILE). */
+int an5d_autotune(const an5d_problem* prob, void* dA, const an5d_layout* layout, int mode,
+                  int max_measured, an5d_config* best, double* best_gstencil);
+int an5d_model_estimate(const an5d_problem* prob, const an5d_config* cfg, double* gstencil);
+int an5d_tune_load(const char* path);
+int an5d_tune_save(const char* path);
+int an5d_tune_clear(void);
+
 /* Release cached device buffers. */
 int an5d_release(void);
 

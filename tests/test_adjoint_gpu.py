@@ -30,7 +30,7 @@ def cases():
         for dt in (np.float32, np.float64):
             adj_bt = {1: 4, 2: 2}.get(O.info(name)["rad"], 1)
             if not O.info(name)["linear"]:
-                adj_bt = 4 if dt == np.float32 else 3
+                adj_bt = 2
             elif O.info(name)["rad"] == 1 and O.info(name)["ntaps"] > 5:
                 adj_bt = 3
             for bt in sorted({1, adj_bt}):
@@ -103,3 +103,49 @@ def test_adjoint_device_entry(torch_cuda):
     hl, hg, hgc, _, _ = pyan5d.adjoint_host(name, u0, target, n, T, coefs=c, divisor=div)
     assert np.array_equal(tg.cpu().numpy(), hg)
     assert np.allclose(gc.cpu().numpy(), hgc, rtol=1e-12, atol=0)
+
+
+def test_autograd_function_gradcheck(torch_cuda):
+    """torch.autograd.gradcheck (central differences, FP64) on a small grid,
+    for both the initial state and the coefficients."""
+    torch = torch_cuda
+    from pyan5d.autograd import sweep
+    for name in ("j2d9pt-gol", "gradient2d"):
+        d = O.info(name)
+        n, T = [12, 14], 5
+        shape = [x + 2 * d["rad"] for x in n]
+        gen = torch.Generator(device="cuda").manual_seed(0)
+        u0 = torch.rand(shape, dtype=torch.float64, device="cuda", generator=gen).requires_grad_()
+        if d["linear"]:
+            c = torch.rand(d["ntaps"], dtype=torch.float64, device="cuda", generator=gen) / d["ntaps"]
+            div = 0.0
+        else:
+            c = torch.tensor([1e-2, 0.5], dtype=torch.float64, device="cuda")
+            div = -1.0
+        c.requires_grad_()
+        mask = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        r = d["rad"]
+        mask[r:-r, r:-r] = 1.0
+        fn = lambda u, cc: sweep(u, cc, name, n, T, div) * mask  # noqa: E731
+        print(f"case gradcheck {name}", flush=True)
+        # the halo is a boundary condition: perturb only interior cells
+        assert torch.autograd.gradcheck(lambda ui, cc: fn(u0.detach() * (1 - mask) + ui * mask, cc),
+                                        (u0, c), eps=1e-6, atol=1e-7, rtol=1e-5, nondet_tol=1e-10)
+
+
+def test_autograd_matches_oracle_fp32(torch_cuda):
+    torch = torch_cuda
+    from pyan5d.autograd import sweep
+    name, n, T = "j2d9pt", [40, 300], 9
+    u0, target, c, div = problem(name, n, np.float32, seed=4)
+    tu = torch.from_numpy(u0).cuda().requires_grad_()
+    tc = torch.from_numpy(c).cuda().requires_grad_()
+    uT = sweep(tu, tc, name, n, T, div)
+    r = O.info(name)["rad"]
+    loss = 0.5 * ((uT - torch.from_numpy(target).cuda())[r:-r, r:-r] ** 2).sum()
+    print("case autograd vs oracle", flush=True)
+    loss.backward()
+    ol, og, ogc = O.adjoint(name, u0.astype(np.float64), target.astype(np.float64), n, T, coefs=c, divisor=div)
+    assert abs(loss.item() - ol) <= 1e-4 * ol
+    assert np.max(np.abs(tu.grad.cpu().numpy() - og)) <= 2e-4 * np.max(np.abs(og))
+    assert np.max(np.abs(tc.grad.cpu().numpy() - ogc)) <= 2e-4 * np.max(np.abs(ogc))

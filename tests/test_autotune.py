@@ -1,0 +1,48 @@
+"""Autotuner: the analytic prior (CPU) and the measured search (GPU)."""
+import numpy as np
+import pytest
+
+import pyan5d
+
+
+def test_model_prefers_sensible_configs():
+    # deeper temporal blocks raise the estimate of a bandwidth-bound 2D star
+    g1 = pyan5d.model_estimate("j2d5pt", [16384, 16384], bt=1, variant=0)
+    g4 = pyan5d.model_estimate("j2d5pt", [16384, 16384], bt=4, variant=0)
+    assert g4 > g1 > 0
+    # a grid far too small to fill the GPU is estimated lower per update
+    small = pyan5d.model_estimate("j2d5pt", [256, 256], bt=4, variant=0)
+    assert small < g4
+    # FMA mode costs fewer instructions
+    gf = pyan5d.model_estimate("j2d5pt", [16384, 16384], bt=8, variant=0, mode=pyan5d.FAST)
+    ge = pyan5d.model_estimate("j2d5pt", [16384, 16384], bt=8, variant=0, mode=pyan5d.EXACT)
+    assert gf > ge
+
+
+def test_tune_table_roundtrip(tmp_path):
+    path = str(tmp_path / "t.tsv")
+    (tmp_path / "t.tsv").write_text("# header\nj2d5pt 0 0 1000 1000 0 0 3 64 1234.5\nnope 0 0 1 1 0 0 1 0 1\n")
+    pyan5d.tune_clear()
+    pyan5d.tune_load(path)
+    out = str(tmp_path / "u.tsv")
+    pyan5d.tune_save(out)
+    lines = [l for l in open(out) if not l.startswith("#")]
+    assert lines == ["j2d5pt 0 0 1000 1000 0 0 3 64 1234.5\n"]
+    pyan5d.tune_clear()
+
+
+@pytest.mark.gpu
+def test_autotune_picks_a_measured_config_and_results_stay_exact(torch_cuda):
+    import oracle_lib as O
+    print("case autotune j2d5pt 600x2000", flush=True)
+    pyan5d.tune_clear()
+    cfg, gs = pyan5d.autotune("j2d5pt", [600, 2000], max_measured=8)
+    assert gs > 0 and cfg["bt"] >= 1
+    rng = np.random.default_rng(0)
+    u = rng.random((602, 2002)).astype(np.float32)
+    A = np.stack([u, u])
+    B = A.copy()
+    pyan5d.forward_host("j2d5pt", A, [600, 2000], 21)  # uses the tuned entry
+    O.run("j2d5pt", B, [600, 2000], 21)
+    assert np.array_equal(A[1], B[1])
+    pyan5d.tune_clear()

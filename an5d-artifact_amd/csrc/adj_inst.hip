@@ -13,7 +13,7 @@ using D = Def<SID>;
 // cells per lane and deepest block: keeps fr/ar rings under ~200 VGPRs
 constexpr int adj_ck() {
   if (!D::linear) return 2;
-  if (D::rad == 1 && D::ntaps <= 5 && !AN5D_F64) return 4;
+  if (D::rad == 1 && !AN5D_F64) return 4;
   return D::rad <= 2 ? 2 : 4;  // a lane holds at least R cells
 }
 constexpr int adj_btmax() {
@@ -35,14 +35,15 @@ hipError_t launch_adj(const AdjLaunch& l, dim3 grid, dim3 block, hipStream_t st)
   a.gacc = l.gacc;
   a.D0 = l.D0; a.D2 = l.D2; a.s0 = l.s0;
   a.ntk = l.ntk; a.nseg = l.nseg; a.sl = l.sl; a.SK = l.SK;
+  a.padded = l.padded;
   hipLaunchKernelGGL((adjoint_kernel<SID, T, BT, adj_ck(), 4>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_seed(const void* uT, const void* target, void* lam, double* loss, int D0, int D2,
-                       long long s0, int R, hipStream_t st) {
+                       long long s0, long long stgt, int R, hipStream_t st) {
   hipLaunchKernelGGL((adjoint_seed_kernel<T>), dim3(1024), dim3(256), 0, st, static_cast<const T*>(uT),
-                     static_cast<const T*>(target), static_cast<T*>(lam), loss, D0, D2, s0, R);
+                     static_cast<const T*>(target), static_cast<T*>(lam), loss, D0, D2, s0, stgt, R);
   return hipGetLastError();
 }
 

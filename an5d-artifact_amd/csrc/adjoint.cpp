@@ -40,12 +40,19 @@ static const AdjVariant* adj_variant(int stencil, int dtype) {
 
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
+// Workspace grids use padded rows: kAdjPad cells before column 0 and at
+// least kAdjPad after the last one, so the adjoint kernel's loads need no
+// bounds checks.  The padding is zero (the workspace is zeroed once) and is
+// never written.
+constexpr long long kPad = 320;  // equals kAdjPad in adjoint.hpp
+
 struct AdjPlan {
   const Variant* fwd = nullptr;
   const AdjVariant* adj = nullptr;
   int bt = 0, sl = 0;
   std::vector<int> steps;
   Geom g;
+  long long pitch = 0;  // elements per padded row
   size_t grid_bytes = 0, ws_bytes = 0;
 };
 
@@ -72,7 +79,8 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   an5d_schedule(p->timesteps, bt, ap.steps.data(), n);
   ap.sl = cfg ? cfg->sl : 0;
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
-  ap.grid_bytes = ((size_t)ap.g.area * esz + 255) / 256 * 256;
+  ap.pitch = (ap.g.D2 + 2 * kPad + 63) / 64 * 64;
+  ap.grid_bytes = ((size_t)ap.g.D0 * ap.pitch * esz + 255) / 256 * 256;
   ap.ws_bytes = ap.grid_bytes * (ap.steps.size() + 1 + 2);
   return AN5D_OK;
 }
@@ -95,7 +103,8 @@ static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, d
   const long long nseg = cdiv(n0, sl);
   l.D0 = (int)ap.g.D0;
   l.D2 = (int)ap.g.D2;
-  l.s0 = ap.g.D2;
+  l.s0 = ap.pitch;
+  l.padded = 1;
   l.ntk = (int)ntk;
   l.nseg = (int)nseg;
   l.sl = (int)sl;
@@ -139,10 +148,12 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
   rc = device_coefs(cbytes, &dcoef);
   if (rc) return rc;
   const int L = (int)ap.steps.size();
+  const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   char* base = static_cast<char*>(ws);
-  auto ckpt = [&](int i) { return base + (size_t)i * ap.grid_bytes; };
+  auto ckpt = [&](int i) { return base + (size_t)i * ap.grid_bytes + (size_t)kPad * esz; };
   char* lam[2] = {ckpt(L + 1), ckpt(L + 2)};
   const int nc = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
+  const long long P = ap.pitch;
   hipError_t e;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (ms_fwd) {
@@ -151,11 +162,10 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     (void)hipEventRecord(ev0, st);
   }
   // checkpoint 0 = u0; every other slot gets u0's halo (forward launches write interiors only)
-  e = hipMemcpyAsync(ckpt(0), du0, (size_t)g.area * (p->dtype == AN5D_FLOAT64 ? 8 : 4),
-                     hipMemcpyDeviceToDevice, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(u0)");
+  e = hipMemcpy2DAsync(ckpt(0), P * esz, du0, g.D2 * esz, g.D2 * esz, g.D0, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy2DAsync(u0)");
   for (int i = 1; i <= L; i++) {
-    e = ap.adj->halo(du0, ckpt(i), (int)g.D0, (int)g.D2, g.D2, R, st);
+    e = ap.adj->halo(ckpt(0), ckpt(i), (int)g.D0, (int)g.D2, P, R, st);
     if (e != hipSuccess) return hip_fail(e, "halo copy");
   }
   // forward with a checkpoint at every launch boundary
@@ -167,18 +177,17 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     lp.args.coef = dcoef;
     lp.args.gscale = gscale;
     lp.args.has_div = has_div;
-    lp.args.s0 = g.D2;
-    lp.args.s1 = g.D2;
+    lp.args.s0 = P;
+    lp.args.s1 = P;
     e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
     if (e != hipSuccess) return hip_fail(e, "forward launch");
   }
   if (ms_fwd) (void)hipEventRecord(ev1, st);
-  // seed: lambda_T = u_T - target on the interior, loss
+  // seed: lambda_T = u_T - target on the interior (0 on the halo), loss
   e = hipMemsetAsync(dloss, 0, sizeof(double), st);
   if (e == hipSuccess) e = hipMemsetAsync(dgcoef, 0, sizeof(double) * nc, st);
-  if (e == hipSuccess) e = hipMemsetAsync(lam[1], 0, ap.grid_bytes, st);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-  e = ap.adj->seed(ckpt(L), dtarget, lam[0], dloss, (int)g.D0, (int)g.D2, g.D2, R, st);
+  e = ap.adj->seed(ckpt(L), dtarget, lam[0], dloss, (int)g.D0, (int)g.D2, P, g.D2, R, st);
   if (e != hipSuccess) return hip_fail(e, "adjoint seed");
   // reverse blocks
   int cur = 0;
@@ -200,9 +209,9 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     if (e != hipSuccess) return hip_fail(e, "adjoint launch");
     cur ^= 1;
   }
-  e = hipMemcpyAsync(dgrad, lam[cur], (size_t)g.area * (p->dtype == AN5D_FLOAT64 ? 8 : 4),
-                     hipMemcpyDeviceToDevice, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(grad)");
+  e = hipMemcpy2DAsync(dgrad, g.D2 * esz, lam[cur], P * esz, g.D2 * esz, g.D0,
+                       hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy2DAsync(grad)");
   if (ms_fwd) {
     float f = 0.f;
     (void)hipEventSynchronize(ev1);
@@ -230,6 +239,7 @@ extern "C" int an5d_adjoint_host(const an5d_problem* p, const void* u0, const vo
   if (e == hipSuccess) e = hipMalloc(&dt, gb);
   if (e == hipSuccess) e = hipMalloc(&dg, gb);
   if (e == hipSuccess) e = hipMalloc(&ws, ap.ws_bytes);
+  if (e == hipSuccess) e = hipMemset(ws, 0, ap.ws_bytes);  // the workspace contract: zeroed once
   if (e == hipSuccess) e = hipMalloc(&dsc, sizeof(double) * (nc + 1));
   auto cleanup = [&]() {
     (void)hipFree(du0);

@@ -37,7 +37,12 @@ struct AdjArgs {
   int D0, D2;
   long long s0;                // row pitch of U, Lin, Lout (elements)
   int ntk, nseg, sl, SK;
+  int padded;                  // rows have >= kAdjPad readable cells either side
 };
+
+// Cells of padding the adjoint workspace keeps on both ends of every row, so
+// a wave's loads never need bounds checks (>= the widest tile overrun).
+constexpr int kAdjPad = 320;  // equals kPad in adjoint.cpp
 
 template <class D>
 constexpr int adj_reach() { return D::linear ? D::rad : 2; }
@@ -156,12 +161,15 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
 #pragma unroll
   for (int g = 0; g < NG; g++) { gsum[g] = 0.0; gpart[g] = 0; }
 
-  // Out-of-grid planes and cells load a clamped (real, finite) neighbour:
-  // they only feed cells that are discarded or multiplied by mu == 0.
+  // Out-of-grid planes load a clamped (real, finite) neighbour plane; in the
+  // padded workspace layout the run of every lane is readable (the padding
+  // holds finite values), so loads need no per-lane bounds branch.  Either
+  // way the stand-in values only feed cells that are discarded or multiplied
+  // by mu == 0.
   auto load_row = [&](const T* base, int p, T (&dst)[CK]) {
     p = p < 0 ? 0 : (p >= a.D0 ? a.D0 - 1 : p);
     const T* row = base + (long long)p * a.s0;
-    if (kfull) {
+    if (a.padded || kfull) {
       load_run<T, CK>(dst, row + kl);
     } else {
 #pragma unroll
@@ -234,6 +242,9 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
         const int qa = pU - DLT - A * R;
         const bool pin = qa >= R && qa < a.D0 - R;
         const bool own = qa >= i0 && qa < i0 + sle && n < nit;
+        // mu restricted to the cells this wave owns (0 elsewhere): the
+        // gradient sums run unconditionally instead of behind a branch
+        T mown[CK];
         // mu planes qa-R..qa+R, row-extended
         T mx[P][CK + 2 * XK];
         static_for<0, P>([&](auto pc) {
@@ -253,7 +264,7 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
             });
             out[KK] = (pin && kin[KK]) ? acc : T(0);
           });
-          if (own) {
+          {
             // sum_x mu(x) u_t(x + d_q) over owned cells of plane qa
             T ux[2 * R + 1][CK + 2 * XK];
             static_for<0, 2 * R + 1>([&](auto pc) {
@@ -262,11 +273,11 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
             });
             static_for<0, CK>([&](auto kc) {
               constexpr int KK = decltype(kc)::value;
-              const T mu = kst[KK] ? mx[R][KK + XK] : T(0);
+              mown[KK] = (own && kst[KK]) ? mx[R][KK + XK] : T(0);
               static_for<0, NT>([&](auto qc) {
                 constexpr int Q = decltype(qc)::value;
                 constexpr TapDef tp = D::taps[Q];
-                gpart[Q] = fma_t(mu, ux[R + tp.d0][KK + tp.d2 + XK], gpart[Q]);
+                gpart[Q] = fma_t(mown[KK], ux[R + tp.d0][KK + tp.d2 + XK], gpart[Q]);
               });
             });
           }
@@ -312,11 +323,9 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
               gC[KK] = mu * (T(1) + f * (dn + ds + de + dw));
               gE[KK] = mu * (-f * de);  // to x + (0,1)
               gW[KK] = mu * (-f * dw);  // to x - (0,1)
-              if (own) {
-                const T muo = kst[KK] ? mu : T(0);
-                gpart[0] = gpart[0] + muo * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
-                gpart[1] = gpart[1] + muo * rs;           // d/d scale
-              }
+              mown[KK] = (own && kst[KK]) ? mu : T(0);
+              gpart[0] = gpart[0] + mown[KK] * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
+              gpart[1] = gpart[1] + mown[KK] * rs;           // d/d scale
             }
           });
           T gEx[CK + 2], gWx[CK + 2];
@@ -364,7 +373,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void adjoint_seed_kernel(const T* __restrict__ uT,
                                                            const T* __restrict__ target,
                                                            T* __restrict__ lam, double* __restrict__ loss,
-                                                           int D0, int D2, long long s0, int R) {
+                                                           int D0, int D2, long long s0, long long st,
+                                                           int R) {
   const long long total = (long long)D0 * D2;
   double part = 0.0;
   for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -372,7 +382,7 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(const T* __restrict__
     const int i = (int)(idx / D2), k = (int)(idx % D2);
     const long long x = (long long)i * s0 + k;
     const bool in = i >= R && i < D0 - R && k >= R && k < D2 - R;
-    T d = in ? uT[x] - target[x] : T(0);
+    T d = in ? uT[x] - target[(long long)i * st + k] : T(0);
     lam[x] = d;
     part += 0.5 * (double)d * (double)d;
   }

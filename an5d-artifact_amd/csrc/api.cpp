@@ -137,7 +137,7 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
   a.D0 = (int)g.D0; a.D1 = (int)g.D1; a.D2 = (int)g.D2;
   lp.fn = (bt >= 1 && bt <= v.btmax) ? (mode == AN5D_FAST ? v.fn_fast[bt] : v.fn[bt]) : nullptr;
   if (!lp.fn) { err = "no kernel for bt=" + std::to_string(bt) + " in this variant"; return false; }
-  lp.lds = v.lds;
+  lp.lds = (bt >= 1 && bt <= kMaxBT) ? v.lds[bt] : 0;
   if (v.kind == kDirect) {
     const long long total = cdiv(nk, v.ck) * nj * n0;
     lp.block = dim3(256);
@@ -399,7 +399,7 @@ extern "C" int an5d_variant_get_info(int stencil, int dtype, int variant, an5d_v
   if (variant < 0 || variant >= (int)vs.size() || !out) return fail(AN5D_E_INVALID, "bad variant");
   const Variant* v = vs[variant];
   out->kind = v->kind; out->ck = v->ck; out->cj = v->cj; out->nw = v->nw;
-  out->btmax = v->btmax; out->lds_bytes = (long long)v->lds;
+  out->btmax = v->btmax; out->lds_bytes = (long long)v->lds[v->btmax];
   return AN5D_OK;
 }
 

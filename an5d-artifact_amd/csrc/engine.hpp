@@ -32,7 +32,7 @@ using LaunchFn = hipError_t (*)(const LaunchArgs&, dim3 grid, dim3 block, size_t
 struct Variant {
   int stencil, dtype, kind;
   int ck, cj, nw, btmax;
-  size_t lds;              // dynamic LDS bytes per workgroup
+  size_t lds[kMaxBT + 1];  // dynamic LDS bytes per workgroup, by bt
   LaunchFn fn[kMaxBT + 1];       // fn[bt], bt in [1, btmax], exact mode
   LaunchFn fn_fast[kMaxBT + 1];  // fast (FMA) mode, same shapes
 };
@@ -52,10 +52,11 @@ struct AdjLaunch {
   int D0, D2;
   long long s0;
   int ntk, nseg, sl, SK;
+  int padded;
 };
 using AdjFn = hipError_t (*)(const AdjLaunch&, dim3 grid, dim3 block, hipStream_t);
 using SeedFn = hipError_t (*)(const void* uT, const void* target, void* lam, double* loss, int D0,
-                              int D2, long long s0, int R, hipStream_t);
+                              int D2, long long s0, long long st, int R, hipStream_t);
 using HaloFn = hipError_t (*)(const void* src, void* dst, int D0, int D2, long long s0, int R,
                               hipStream_t);
 

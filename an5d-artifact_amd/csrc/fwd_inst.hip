@@ -53,15 +53,15 @@ hipError_t launch_direct(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds,
 template <int W>
 void add_sweep(std::vector<Variant>& out) {
   constexpr Shape sh = sweep_shape<SID, T>(W);
-  if constexpr (sh.btmax > 0 && sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>() <= 160 * 1024) {
+  if constexpr (sh.btmax > 0 && sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>(sh.btmax) <= 160 * 1024) {
     Variant v{};
     v.stencil = SID;
     v.dtype = AN5D_F64 ? kF64 : kF32;
     v.kind = kSweep;
     v.ck = sh.ck; v.cj = sh.cj; v.nw = sh.nw; v.btmax = sh.btmax;
-    v.lds = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>();
     static_for<1, sh.btmax + 1>([&](auto bc) {
       constexpr int BT = decltype(bc)::value;
+      v.lds[BT] = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>(BT);
       v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true>;
       v.fn_fast[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, false>;
     });
@@ -82,7 +82,7 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   d.dtype = AN5D_F64 ? kF64 : kF32;
   d.kind = kDirect;
   d.ck = direct_ck<SID, T>(); d.cj = 1; d.nw = 4; d.btmax = 1;
-  d.lds = 0;
+  for (auto& l : d.lds) l = 0;
   d.fn[1] = &launch_direct<direct_ck<SID, T>()>;
   d.fn_fast[1] = d.fn[1];
   out.push_back(d);

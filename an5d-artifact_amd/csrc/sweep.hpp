@@ -94,26 +94,29 @@ __device__ __forceinline__ void store_run(T* p, const T (&src)[N]) {
 }
 
 // ---- correctly rounded division by a constant --------------------------------
-// has_div == 2: q0 = x*rc, e = fma(-q0, d, x), q = copysign(fma(e, rc, q0), x)
-// with rc = RN(1/d) (Markstein's correction step, sign taken from x).  When q
-// is +-0 or normal it equals IEEE x/d: checked for all 2^32 floats with
-// d = 118 and d = 159, and for doubles on every multiple of the subnormal
-// quantum below 2^28 quanta plus 1.3e9 random inputs over all exponents.
-// Other q (subnormal, inf, NaN) take the IEEE division.  has_div == 1: IEEE.
+// has_div == 2: q0 = x*rc, e' = fma(q0, d, -x) (exact residual, negated),
+// q = fma(-e', rc, q0) with rc = RN(1/d) (Markstein's correction step; the
+// negated residual makes x = -0 give -0).  When q is +-0 or normal it equals
+// IEEE x/d: checked for all 2^32 floats with d = 118 and d = 159, and for
+// doubles on every multiple of the subnormal quantum below 2^28 quanta plus
+// 1.3e9 random inputs over all exponents.  Other q (subnormal, inf, NaN)
+// take the IEEE division.  has_div == 1: IEEE.
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float copysign_t(float a, float b) { return __builtin_copysignf(a, b); }
-__device__ __forceinline__ double copysign_t(double a, double b) { return __builtin_copysign(a, b); }
 // classes: -normal | -0 | +0 | +normal
 __device__ __forceinline__ bool zero_or_normal(float q) { return __builtin_amdgcn_classf(q, 0x168); }
 __device__ __forceinline__ bool zero_or_normal(double q) { return __builtin_amdgcn_class(q, 0x168); }
 template <typename T>
+__device__ __forceinline__ T div_fast(T x, T d, T rc) {
+  const T q0 = x * rc;
+  const T ep = fma_t(q0, d, -x);
+  return fma_t(-ep, rc, q0);
+}
+template <typename T>
 __device__ __forceinline__ T div_by(T x, T d, T rc, int mode) {
   if (mode == 0) return x;
   if (mode == 2) {
-    const T q0 = x * rc;
-    const T e = fma_t(-q0, d, x);
-    T q = copysign_t(fma_t(e, rc, q0), x);
+    T q = div_fast(x, d, rc);
     // wave-uniform test keeps the IEEE fallback a skipped branch
     const bool ok = zero_or_normal(q);
     if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
@@ -124,8 +127,6 @@ __device__ __forceinline__ T div_by(T x, T d, T rc, int mode) {
   return x / d;
 }
 
-// ---- one stencil evaluation, exact source order ---------------------------
-// Get::at<d0,d1,d2>() returns the stage input at that offset.
 // Fast mode (EXACT == false): tolerance-checked, not bitwise.  Linear
 // stencils take pre-folded coefficients s_q*c_q/div and chain FMAs.  (The
 // nonlinear gradient2d keeps the exact evaluation in both modes.)
@@ -224,11 +225,22 @@ constexpr int edge_index(int rr) {
   return -1;
 }
 
-// LDS bytes of the middle-dim exchange (two buffers).
+// Stars (only the centre plane has in-plane taps, no corners): all stages'
+// middle-dim exchanges of an iteration share one LDS round.
+template <class D>
+constexpr bool star_exchange() {
+  return D::dim == 3 && needs_mid<D>() && num_ext_planes<D>() == 1 && ext_slot<D>(0) == 0 &&
+         !needs_corners<D>();
+}
+
+// LDS bytes of the middle-dim exchange: stars one buffer with a slot per
+// stage, the others two buffers of one slot per extended plane.
 template <int ID, typename T, int CK, int CJ, int NW>
-constexpr size_t sweep_lds_bytes() {
+constexpr size_t sweep_lds_bytes(int bt) {
   using D = Def<ID>;
-  if constexpr (D::dim == 3 && needs_mid<D>())
+  if constexpr (star_exchange<D>())
+    return (size_t)bt * NW * edge_rows<CJ, D::rad>() * 64 * CK * sizeof(T);
+  else if constexpr (D::dim == 3 && needs_mid<D>())
     return 2ull * num_ext_planes<D>() * NW * edge_rows<CJ, D::rad>() * 64 * CK * sizeof(T);
   else
     return 0;
@@ -354,7 +366,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   };
 
   // Stage S output plane from ring[S-1]; q = global index of that plane.
-  auto compute = [&](const Plane<T, CJ, CK>(&rin)[P], Plane<T, CJ, CK>& out, int q) {
+  constexpr bool STARX = star_exchange<D>();
+  constexpr int ERS = edge_rows<CJ, XJ>();
+  T xh[STARX ? BT : 1][STARX ? 2 * XJ : 1][CK];  // halo rows of each stage's centre plane
+
+  auto compute = [&](const Plane<T, CJ, CK>(&rin)[P], Plane<T, CJ, CK>& out, int q, auto sc) {
+    constexpr int S = decltype(sc)::value;
     // extended copies of the planes that need in-plane neighbours
     T x[NX > 0 ? NX : 1][CJ + 2 * XJ][CK + 2 * XK];
     static_for<-R, R + 1>([&](auto dc) {
@@ -376,7 +393,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         }
       }
     });
-    if constexpr (MID) {
+    if constexpr (STARX) {
+#pragma unroll
+      for (int h = 0; h < 2 * XJ; h++) {
+        const int jr = h < XJ ? h : CJ + XJ + (h - XJ);  // x row index of the halo row
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) x[0][jr][kk + XK] = xh[S - 1][h][kk];
+      }
+    } else if constexpr (MID) {
       // publish own edge rows, then fetch the XJ rows above and below
       constexpr int ER = edge_rows<CJ, XJ>();
       T* lds = reinterpret_cast<T*>(an5d_lds) + (size_t)xbuf * NX * NW * ER * 64 * CK;
@@ -422,15 +446,39 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
     }
     const bool pin = q >= R && q < a.D0 - R;
     T val[CJ][CK];
+    // DM == 2: the tap sums of the whole plane first, then the divisions with
+    // one wave-uniform guard for all of them
+    constexpr bool BATCHDIV = EXACT && D::linear && DM == 2;
     static_for<0, CJ>([&](auto jc) {
       constexpr int JJ = decltype(jc)::value;
       static_for<0, CK>([&](auto kc) {
         constexpr int KK = decltype(kc)::value;
         RegGet<D, T, (NX > 0 ? NX : 1), CJ, CK, XJ, XK, JJ, KK> get{x, rin};
-        if constexpr (EXACT || !D::linear) val[JJ][KK] = eval_point<D, T, DM>(get, c, div, has_div, gscale);
+        if constexpr (BATCHDIV) val[JJ][KK] = eval_point<D, T, 0>(get, c, div, has_div, gscale);
+        else if constexpr (EXACT || !D::linear) val[JJ][KK] = eval_point<D, T, DM>(get, c, div, has_div, gscale);
         else val[JJ][KK] = eval_fast<D, T>(get, c);
       });
     });
+    if constexpr (BATCHDIV) {
+      const T rc = c[NT + 1];
+      T sum[CJ][CK];
+      bool ok = true;
+#pragma unroll
+      for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) {
+          sum[jj][kk] = val[jj][kk];
+          val[jj][kk] = div_fast(sum[jj][kk], div, rc);
+          ok = ok && zero_or_normal(val[jj][kk]);
+        }
+      if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+#pragma unroll
+        for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++)
+            if (!zero_or_normal(val[jj][kk])) val[jj][kk] = sum[jj][kk] / div;
+      }
+    }
     if (pin && wave_in) {
 #pragma unroll
       for (int jj = 0; jj < CJ; jj++)
@@ -457,6 +505,38 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
     for (int u = 0; u < UNR; u++) {
       const int n = n0 + u;
       const int p_in = i0 - H + n;
+      if constexpr (STARX) {
+        // Every stage's centre plane this iteration is slot R+1 of its input
+        // ring before the rotation: publish their edge rows, one barrier,
+        // read the neighbours' rows, a second barrier before the next write.
+        T* lds = reinterpret_cast<T*>(an5d_lds);
+        static_for<1, BT + 1>([&](auto sc) {
+          constexpr int S = decltype(sc)::value;
+          static_for<0, CJ>([&](auto jc) {
+            constexpr int JJ = decltype(jc)::value;
+            constexpr int E = edge_index<CJ, XJ>(JJ);
+            if constexpr (E >= 0)
+              store_run<T, CK>(lds + ((((size_t)(S - 1) * NW + w) * ERS + E) * 64 + lane) * CK,
+                               ring[S - 1][R + 1].v[JJ]);
+          });
+        });
+        __syncthreads();
+        static_for<1, BT + 1>([&](auto sc) {
+          constexpr int S = decltype(sc)::value;
+          static_for<0, 2 * XJ>([&](auto hc) {
+            constexpr int HI = decltype(hc)::value;
+            constexpr int JR = HI < XJ ? HI - XJ : CJ + (HI - XJ);
+            constexpr int WOFF = JR >= 0 ? JR / CJ : -((-JR + CJ - 1) / CJ);
+            constexpr int RR = JR - WOFF * CJ;
+            constexpr int E = edge_index<CJ, XJ>(RR);
+            static_assert(E >= 0, "halo row must be published");
+            int ws = w + WOFF;
+            ws = ws < 0 ? 0 : (ws >= NW ? NW - 1 : ws);
+            load_run<T, CK>(xh[S - 1][HI], lds + ((((size_t)(S - 1) * NW + ws) * ERS + E) * 64 + lane) * CK);
+          });
+        });
+        __syncthreads();
+      }
 #pragma unroll
       for (int p = 0; p < P - 1; p++) ring[0][p] = ring[0][p + 1];
       ring[0][P - 1] = pre;
@@ -464,7 +544,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       static_for<1, BT + 1>([&](auto sc) {
         constexpr int S = decltype(sc)::value;
         Plane<T, CJ, CK> out;
-        compute(ring[S - 1], out, p_in - S * R);
+        compute(ring[S - 1], out, p_in - S * R, sc);
         if constexpr (S < BT) {
 #pragma unroll
           for (int p = 0; p < P - 1; p++) ring[S][p] = ring[S][p + 1];

@@ -307,7 +307,7 @@ class AdjointWorkspace:
         nbytes = ctypes.c_size_t()
         _check(lib().an5d_adjoint_workspace(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(nbytes)))
         self.nbytes = nbytes.value
-        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+        self.buf = torch.zeros(self.nbytes, dtype=torch.uint8, device="cuda")  # zeroed once
 
 
 def adjoint_device(name, u0, target, grad, gcoef, loss, n, T, ws, coefs=None, divisor=-1.0,

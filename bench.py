@@ -211,6 +211,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+adj and batched legs")
     ap.add_argument("--mode", type=int, default=0, help="0 exact (bitwise), 1 fast (FMA, tolerance)")
+    ap.add_argument("--timesteps", type=int, default=0,
+                    help="override the workload's timesteps (short profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (scale reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -228,6 +230,8 @@ def main():
 
     wl = WORKLOADS[args.workload]
     name, dt, n, T = wl["stencil"], wl["dtype"], wl["n"], wl["T"]
+    if args.timesteps > 0:
+        T = args.timesteps
     d = pyan5d.info(name)
     coefs, divisor = presets.coefficients(name, "baseline")
     dcode = 0 if dt == np.float32 else 1

@@ -157,7 +157,9 @@ int an5d_init_grid(int dtype, void* A, long long cells, long long seed_base, int
 int an5d_adjoint_workspace(const an5d_problem* prob, const an5d_config* cfg, size_t* bytes);
 /* Device form: enqueues on `stream`; *ms_fwd (optional) gets the time of the
  * checkpointed forward pass (synchronises on an event when non-NULL).
- * gcoef and loss are device doubles. */
+ * gcoef and loss are device doubles.  The workspace (size from
+ * an5d_adjoint_workspace) must be zero-filled once before its first use;
+ * the library keeps the parts it relies on zero afterwards. */
 int an5d_adjoint_device(const an5d_problem* prob, const void* u0, const void* target, void* grad,
                         double* gcoef, double* loss, void* workspace, size_t workspace_bytes,
                         const an5d_config* cfg, void* stream, double* ms_fwd);

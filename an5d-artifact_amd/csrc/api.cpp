@@ -320,6 +320,26 @@ int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout,
 
 using namespace an5d;
 
+extern "C" int an5d_resolve_config(const an5d_problem* p, const an5d_config* cfg,
+                                   an5d_config* out) {
+  if (!p || !out) return fail(AN5D_E_INVALID, "null argument");
+  Geom g;
+  std::string err;
+  if (!make_geom(p, g, err)) return fail(AN5D_E_INVALID, err);
+  const Variant* v = nullptr;
+  int bt = 0, sl = 0;
+  int rc = resolve(p, cfg, &v, &bt, &sl);
+  if (rc) return rc;
+  const int mode = cfg ? cfg->mode : AN5D_EXACT;
+  LaunchPlan lp;
+  if (!plan_launch(*v, bt, g, sl, lp, err, mode)) return fail(AN5D_E_UNSUPPORTED, err);
+  auto vs = variants_of(p->stencil, p->dtype);
+  int vi = 0;
+  while (vi < (int)vs.size() && vs[vi] != v) vi++;
+  *out = an5d_config{bt, vi, v->kind == kDirect ? 0 : lp.args.sl, mode};
+  return AN5D_OK;
+}
+
 // ---- schedule -------------------------------------------------------------------
 extern "C" int an5d_schedule(int T, int bt, int* steps, int max_launches) {
   if (T < 0 || bt < 1) return fail(AN5D_E_INVALID, "schedule needs T >= 0 and bt >= 1");

@@ -94,6 +94,7 @@ def lib():
         L.an5d_autotune.argtypes = [P(Problem), ctypes.c_void_p, P(Layout), ctypes.c_int, ctypes.c_int,
                                     P(Config), P(ctypes.c_double)]
         L.an5d_model_estimate.argtypes = [P(Problem), P(Config), P(ctypes.c_double)]
+        L.an5d_resolve_config.argtypes = [P(Problem), P(Config), P(Config)]
         L.an5d_tune_load.argtypes = [ctypes.c_char_p]
         L.an5d_tune_save.argtypes = [ctypes.c_char_p]
         L.an5d_cpu_forward.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double)]
@@ -385,6 +386,16 @@ def model_estimate(name, n, dtype=np.float32, bt=1, variant=0, sl=0, mode=EXACT)
     gs = ctypes.c_double()
     _check(lib().an5d_model_estimate(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(gs)))
     return gs.value
+
+
+def resolve_config(name, n, dtype=np.float32, bt=0, variant=-1, sl=0, mode=EXACT):
+    """The (variant, bt, sl) the forward entry points run for this problem:
+    explicit values win, then the tuned table, then the defaults."""
+    p, keep = make_problem(name, n, 1, dtype)
+    cfg = make_config(bt, variant, sl, mode)
+    out = Config()
+    _check(lib().an5d_resolve_config(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(out)))
+    return dict(bt=out.bt, variant=out.variant, sl=out.sl, mode=out.mode)
 
 
 def tune_save(path=None):

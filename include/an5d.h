@@ -196,6 +196,12 @@ int an5d_batch_destroy(an5d_batch* batch);
 int an5d_autotune(const an5d_problem* prob, void* dA, const an5d_layout* layout, int mode,
                   int max_measured, an5d_config* best, double* best_gstencil);
 int an5d_model_estimate(const an5d_problem* prob, const an5d_config* cfg, double* gstencil);
+/* The configuration the forward entry points would run for `prob` and `cfg`
+ * (NULL: all defaults): the variant, bt and sl after the tuned-table lookup
+ * and the per-stencil defaults, with sl = the planes per segment actually
+ * launched (0 for one-step kernels).  The reference fixes these at compile
+ * time per binary (compiled/nvcc_compile_top5.sh); this reports the choice. */
+int an5d_resolve_config(const an5d_problem* prob, const an5d_config* cfg, an5d_config* out);
 int an5d_tune_load(const char* path);
 int an5d_tune_save(const char* path);
 int an5d_tune_clear(void);

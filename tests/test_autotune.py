@@ -46,3 +46,28 @@ def test_autotune_picks_a_measured_config_and_results_stay_exact(torch_cuda):
     O.run("j2d5pt", B, [600, 2000], 21)
     assert np.array_equal(A[1], B[1])
     pyan5d.tune_clear()
+
+
+def test_resolve_config_follows_flags_then_table_then_defaults(tmp_path):
+    pyan5d.tune_clear()
+    # defaults: first variant, per-stencil bt, sl sized by the library
+    d = pyan5d.resolve_config("j2d5pt", [1000, 1000])
+    assert d["variant"] == 0 and d["bt"] == 4 and d["sl"] >= 1 and d["mode"] == pyan5d.EXACT
+    # explicit values win
+    e = pyan5d.resolve_config("j2d5pt", [1000, 1000], bt=3, variant=0, sl=48)
+    assert (e["variant"], e["bt"], e["sl"]) == (0, 3, 48)
+    # a tuned entry is used for sizes within 8x; its sl only for the exact size
+    (tmp_path / "t.tsv").write_text("j2d5pt 0 0 1000 1000 0 0 6 64 1234.5\n")
+    pyan5d.tune_load(str(tmp_path / "t.tsv"))
+    t = pyan5d.resolve_config("j2d5pt", [1000, 1000])
+    assert (t["variant"], t["bt"], t["sl"]) == (0, 6, 64)
+    near = pyan5d.resolve_config("j2d5pt", [1200, 1000])
+    assert near["bt"] == 6
+    assert near["sl"] == pyan5d.resolve_config("j2d5pt", [1200, 1000], bt=6, variant=0)["sl"]
+    far = pyan5d.resolve_config("j2d5pt", [8000, 8000])
+    assert far["bt"] == 4
+    # fast mode has its own rows
+    assert pyan5d.resolve_config("j2d5pt", [1000, 1000], mode=pyan5d.FAST)["bt"] == 4
+    pyan5d.tune_clear()
+    with pytest.raises(pyan5d.An5dError):
+        pyan5d.resolve_config("j2d5pt", [1000, 1000], variant=99)

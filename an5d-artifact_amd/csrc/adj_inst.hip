@@ -32,18 +32,17 @@ hipError_t launch_adj(const AdjLaunch& l, dim3 grid, dim3 block, hipStream_t st)
   a.coef = static_cast<const T*>(l.coef);
   a.gscale = l.gscale;
   a.has_div = l.has_div;
-  a.gacc = l.gacc;
+  a.gslot = l.gslot;
   a.D0 = l.D0; a.D2 = l.D2; a.s0 = l.s0;
   a.ntk = l.ntk; a.nseg = l.nseg; a.sl = l.sl; a.SK = l.SK;
-  a.padded = l.padded;
   hipLaunchKernelGGL((adjoint_kernel<SID, T, BT, adj_ck(), 4>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_seed(const void* uT, const void* target, void* lam, double* loss, int D0, int D2,
+hipError_t launch_seed(const void* uT, const void* target, void* lam, double* lslot, int D0, int D2,
                        long long s0, long long stgt, int R, hipStream_t st) {
-  hipLaunchKernelGGL((adjoint_seed_kernel<T>), dim3(1024), dim3(256), 0, st, static_cast<const T*>(uT),
-                     static_cast<const T*>(target), static_cast<T*>(lam), loss, D0, D2, s0, stgt, R);
+  hipLaunchKernelGGL((adjoint_seed_kernel<T>), dim3(kSeedBlocks), dim3(256), 0, st, static_cast<const T*>(uT),
+                     static_cast<const T*>(target), static_cast<T*>(lam), lslot, D0, D2, s0, stgt, R);
   return hipGetLastError();
 }
 

@@ -40,11 +40,12 @@ static const AdjVariant* adj_variant(int stencil, int dtype) {
 
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
-// Workspace grids use padded rows: kAdjPad cells before column 0 and at
-// least kAdjPad after the last one, so the adjoint kernel's loads need no
-// bounds checks.  The padding is zero (the workspace is zeroed once) and is
-// never written.
-constexpr long long kPad = 320;  // equals kAdjPad in adjoint.hpp
+// Workspace grids are padded: kAdjPad cells before column 0 and at least
+// kAdjPad after the last one, kAdjRowPad rows above row 0 and below the last,
+// so the adjoint kernel's loads need no bounds checks.  The padding is zero
+// (the workspace is zeroed once) and is never written.
+constexpr long long kPad = kAdjPad;
+constexpr long long kRowPad = kAdjRowPad;
 
 struct AdjPlan {
   const Variant* fwd = nullptr;
@@ -54,7 +55,14 @@ struct AdjPlan {
   Geom g;
   long long pitch = 0;  // elements per padded row
   size_t grid_bytes = 0, ws_bytes = 0;
+  // after the grids: per-wave gradient slots (nslots x ng doubles), then the
+  // seed kernel's per-wave loss slots (kSeedBlocks * 4 doubles)
+  long long nslots = 0;
+  int ng = 0;
+  size_t gslot_off = 0, lslot_off = 0;
 };
+
+static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block);
 
 static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& ap) {
   std::string err;
@@ -80,9 +88,58 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   ap.sl = cfg ? cfg->sl : 0;
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   ap.pitch = (ap.g.D2 + 2 * kPad + 63) / 64 * 64;
-  ap.grid_bytes = ((size_t)ap.g.D0 * ap.pitch * esz + 255) / 256 * 256;
-  ap.ws_bytes = ap.grid_bytes * (ap.steps.size() + 1 + 2);
+  ap.grid_bytes = ((size_t)(ap.g.D0 + 2 * kRowPad) * ap.pitch * esz + 255) / 256 * 256;
+  // gradient slots: the most wave tiles any launch of the schedule uses
+  ap.ng = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
+  ap.nslots = 0;
+  for (int k : ap.steps) {
+    AdjLaunch l{};
+    dim3 grid, block;
+    int rc = adj_launch_plan(ap, k, l, grid, block);
+    if (rc) return rc;
+    const long long n = (long long)l.ntk * l.nseg;
+    if (n > ap.nslots) ap.nslots = n;
+  }
+  ap.gslot_off = ap.grid_bytes * (ap.steps.size() + 1 + 2);
+  ap.lslot_off = ap.gslot_off + ((size_t)ap.nslots * ap.ng * sizeof(double) + 255) / 256 * 256;
+  ap.ws_bytes = ap.lslot_off + (size_t)kSeedBlocks * 4 * sizeof(double);
   return AN5D_OK;
+}
+
+// Zeroes n doubles (a fill kernel rather than a memset node, so the call
+// sequence stays capturable in a graph).
+__global__ __launch_bounds__(256) void zero_f64_kernel(double* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// Block b < ng: gacc[b] = sum over the gradient slots of entry b; block ng:
+// loss = sum of the loss slots.  Fixed summation order: the results are
+// reproducible run to run.
+__global__ __launch_bounds__(256) void adj_finish_kernel(const double* __restrict__ gslot,
+                                                         long long nslots, int ng,
+                                                         double* __restrict__ gacc,
+                                                         const double* __restrict__ lslot, int nl,
+                                                         double* __restrict__ loss) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  double s = 0.0;
+  if (b < ng) {
+    for (long long i = threadIdx.x; i < nslots; i += 256) s += gslot[i * ng + b];
+  } else {
+    for (int i = threadIdx.x; i < nl; i += 256) s += lslot[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (b < ng) gacc[b] = red[0];
+    else *loss = red[0];
+  }
 }
 
 static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block) {
@@ -104,7 +161,17 @@ static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, d
   l.D0 = (int)ap.g.D0;
   l.D2 = (int)ap.g.D2;
   l.s0 = ap.pitch;
-  l.padded = 1;
+  // planes the kernel reads (see adjoint_kernel): checkpoint planes from
+  // i0-HU on, lambda planes from i0-HU-DLT on (HU = 2(k-1)R+W,
+  // DLT = W+(k-2)R), through i0+sle+DLT+k*R-1 plus the unrolled tail and the
+  // prefetch queue (< 2 ring lengths, a ring length <= 2W+(2k-2)R+2); all of
+  // them must stay inside the padding rows
+  const long long HU = 2LL * (k - 1) * R + W, DLT = W + (long long)(k - 2) * R;
+  const long long ring = 2LL * W + (2LL * k - 2) * R + 2;
+  const long long lo = R - HU - (DLT > 0 ? DLT : 0);
+  const long long hi = (ap.g.D0 - R) + (DLT > 0 ? DLT : 0) + (long long)k * R + 2 * ring;
+  if (lo < -kRowPad || hi >= ap.g.D0 + kRowPad || 64LL * ap.adj->ck > kPad)
+    return fail(AN5D_E_UNSUPPORTED, "adjoint block reaches beyond the workspace padding");
   l.ntk = (int)ntk;
   l.nseg = (int)nseg;
   l.sl = (int)sl;
@@ -150,9 +217,12 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
   const int L = (int)ap.steps.size();
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   char* base = static_cast<char*>(ws);
-  auto ckpt = [&](int i) { return base + (size_t)i * ap.grid_bytes + (size_t)kPad * esz; };
+  auto ckpt = [&](int i) {
+    return base + (size_t)i * ap.grid_bytes + (size_t)(kRowPad * ap.pitch + kPad) * esz;
+  };
   char* lam[2] = {ckpt(L + 1), ckpt(L + 2)};
-  const int nc = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
+  double* gslot = reinterpret_cast<double*>(base + ap.gslot_off);
+  double* lslot = reinterpret_cast<double*>(base + ap.lslot_off);
   const long long P = ap.pitch;
   hipError_t e;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -183,11 +253,14 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     if (e != hipSuccess) return hip_fail(e, "forward launch");
   }
   if (ms_fwd) (void)hipEventRecord(ev1, st);
-  // seed: lambda_T = u_T - target on the interior (0 on the halo), loss
-  e = hipMemsetAsync(dloss, 0, sizeof(double), st);
-  if (e == hipSuccess) e = hipMemsetAsync(dgcoef, 0, sizeof(double) * nc, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-  e = ap.adj->seed(ckpt(L), dtarget, lam[0], dloss, (int)g.D0, (int)g.D2, P, g.D2, R, st);
+  // seed: lambda_T = u_T - target on the interior (0 on the halo), loss slots
+  {
+    const long long nz = (long long)(ap.ws_bytes - ap.gslot_off) / (long long)sizeof(double);
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(256), dim3(256), 0, st, gslot, nz);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "slot clear");
+  }
+  e = ap.adj->seed(ckpt(L), dtarget, lam[0], lslot, (int)g.D0, (int)g.D2, P, g.D2, R, st);
   if (e != hipSuccess) return hip_fail(e, "adjoint seed");
   // reverse blocks
   int cur = 0;
@@ -203,7 +276,7 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     l.coef = dcoef;
     l.gscale = gscale;
     l.has_div = has_div;
-    l.gacc = dgcoef;
+    l.gslot = gslot;
     if (!ap.adj->fn[k]) return fail(AN5D_E_UNSUPPORTED, "no adjoint kernel for this block size");
     e = ap.adj->fn[k](l, grid, block, st);
     if (e != hipSuccess) return hip_fail(e, "adjoint launch");
@@ -212,6 +285,10 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
   e = hipMemcpy2DAsync(dgrad, g.D2 * esz, lam[cur], P * esz, g.D2 * esz, g.D0,
                        hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy2DAsync(grad)");
+  hipLaunchKernelGGL(adj_finish_kernel, dim3(ap.ng + 1), dim3(256), 0, st, gslot, ap.nslots, ap.ng,
+                     dgcoef, lslot, kSeedBlocks * 4, dloss);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "gradient reduction");
   if (ms_fwd) {
     float f = 0.f;
     (void)hipEventSynchronize(ev1);
@@ -232,7 +309,7 @@ extern "C" int an5d_adjoint_host(const an5d_problem* p, const void* u0, const vo
   if (!u0 || !target || !grad || !gcoef || !loss) return fail(AN5D_E_INVALID, "null host pointer");
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   const size_t gb = (size_t)ap.g.area * esz;
-  const int nc = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
+  const int nc = ap.ng;
   void *du0 = nullptr, *dt = nullptr, *dg = nullptr, *ws = nullptr;
   double* dsc = nullptr;
   hipError_t e = hipMalloc(&du0, gb);

@@ -21,6 +21,7 @@
 // FMA; the recomputed forward states are exact.
 #pragma once
 
+#include "engine.hpp"
 #include "sweep.hpp"
 
 namespace an5d {
@@ -33,16 +34,23 @@ struct AdjArgs {
   const T* __restrict__ coef;  // forward coefficients, then the divisor (gradient2d: eps)
   double gscale;
   int has_div;
-  double* __restrict__ gacc;   // coefficient-gradient accumulators
+  double* __restrict__ gslot;  // per-wave coefficient-gradient sums: NG doubles per wave tile
   int D0, D2;
   long long s0;                // row pitch of U, Lin, Lout (elements)
   int ntk, nseg, sl, SK;
-  int padded;                  // rows have >= kAdjPad readable cells either side
 };
 
-// Cells of padding the adjoint workspace keeps on both ends of every row, so
-// a wave's loads never need bounds checks (>= the widest tile overrun).
-constexpr int kAdjPad = 320;  // equals kPad in adjoint.cpp
+// Input rows in flight ahead of their use: the largest divisor of the ring
+// length that fits ~28 VGPRs (the queue then rotates by renaming), else 2 with
+// the ring length rounded up to even.
+template <int LFMAX, int CK, int ESZ>
+constexpr int adj_prefetch() {
+  const int per = 2 * CK * ESZ / 4;  // VGPRs per queued plane (U and lambda rows)
+  int best = 1;
+  for (int d = 2; d <= LFMAX; d++)
+    if (LFMAX % d == 0 && d * per <= 28) best = d;
+  return best > 1 ? best : 2;
+}
 
 template <class D>
 constexpr int adj_reach() { return D::linear ? D::rad : 2; }
@@ -119,7 +127,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
     kst[kk] = k >= kv0 && k < kv1;
     kst_all = kst_all && kst[kk];
   }
-  const bool kfull = kl >= 0 && kl + CK <= a.D2;
 
   T c[NT + 2];
 #pragma unroll
@@ -144,7 +151,8 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   // dead and cost nothing.  Rings start at zero and every load reads a real
   // grid cell, so every value the pipeline computes is finite and the
   // masked products (mu == 0 off the interior) stay exactly zero.
-  constexpr int L = LFMAX;
+  constexpr int PF = adj_prefetch<LFMAX, CK, (int)sizeof(T)>();
+  constexpr int L = (LFMAX + PF - 1) / PF * PF;
   T fr[BT][L][CK];  // forward rings (stage 0 = checkpoint input)
   T ar[BT][L][CK];  // adjoint rings (stage 0 = lambda input)
 #pragma unroll
@@ -161,25 +169,10 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
 #pragma unroll
   for (int g = 0; g < NG; g++) { gsum[g] = 0.0; gpart[g] = 0; }
 
-  // Out-of-grid planes load a clamped (real, finite) neighbour plane; in the
-  // padded workspace layout the run of every lane is readable (the padding
-  // holds finite values), so loads need no per-lane bounds branch.  Either
-  // way the stand-in values only feed cells that are discarded or multiplied
-  // by mu == 0.
-  auto load_row = [&](const T* base, int p, T (&dst)[CK]) {
-    p = p < 0 ? 0 : (p >= a.D0 ? a.D0 - 1 : p);
-    const T* row = base + (long long)p * a.s0;
-    if (a.padded || kfull) {
-      load_run<T, CK>(dst, row + kl);
-    } else {
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) {
-        int k = kl + kk;
-        k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
-        dst[kk] = row[k];
-      }
-    }
-  };
+  // Planes outside the grid read the workspace's zero padding rows and the
+  // lanes beyond the row ends its zero padding cells, so every value the
+  // pipeline computes is finite and the masked products (mu == 0 off the
+  // interior) stay exactly zero.
   auto shift = [&](T (&ring)[L][CK], const T (&in)[CK]) {
 #pragma unroll
     for (int p = 0; p < L - 1; p++)
@@ -189,20 +182,35 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
     for (int kk = 0; kk < CK; kk++) ring[L - 1][kk] = in[kk];
   };
 
-  // inputs are fetched one iteration ahead
-  T preU[CK], preL[CK];
-  load_row(a.U, i0 - HU, preU);
-  load_row(a.Lin, i0 - HU - DLT, preL);
+  // inputs are fetched PF iterations ahead: iteration n consumes checkpoint
+  // plane i0-HU+n and lambda plane i0-HU-DLT+n
+  const long long s0 = a.s0;
+  // uniform row pointers biased by -kAdjPad and a non-negative lane offset
+  const unsigned koff = (unsigned)(kl + kAdjPad);
+  const T* up = a.U + (long long)(i0 - HU) * s0 - kAdjPad;
+  const T* lp = a.Lin + (long long)(i0 - HU - DLT) * s0 - kAdjPad;
+  T preU[PF][CK], preL[PF][CK];
+#pragma unroll
+  for (int f = 0; f < PF; f++) {
+    load_run<T, CK>(preU[f], up + koff);
+    load_run<T, CK>(preL[f], lp + koff);
+    up += s0;
+    lp += s0;
+  }
   const int nit_pad = (nit + L - 1) / L * L;
   for (int n0 = 0; n0 < nit_pad; n0 += L) {
 #pragma unroll
     for (int u = 0; u < L; u++) {
       const int n = n0 + u;
       const int pU = i0 - HU + n;
-      shift(fr[0], preU);
-      shift(ar[0], preL);
-      load_row(a.U, pU + 1, preU);
-      load_row(a.Lin, pU + 1 - DLT, preL);
+      shift(fr[0], preU[u % PF]);
+      shift(ar[0], preL[u % PF]);
+      // unconditional (the padding rows absorb the overrun): a conditional
+      // load would merge with the old slot value and wait for the load at once
+      load_run<T, CK>(preU[u % PF], up + koff);
+      load_run<T, CK>(preL[u % PF], lp + koff);
+      up += s0;
+      lp += s0;
       // ---- forward recompute (stages 1..BT-1) ------------------------------
       static_for<1, BT>([&](auto mc) {
         constexpr int M = decltype(mc)::value;
@@ -364,15 +372,19 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
       // d u_{t+1} / d c_q = s_q u_t(x + d_q) / div
       v = (D::taps[G].sign > 0 ? v : -v) / (has_div ? (double)div : 1.0);
     }
-    if (lane == 0) atomicAdd(a.gacc + G, v);
+    // each wave tile owns its slot (no atomics: thousands of waves ending
+    // together on the same few addresses would serialise); a separate
+    // reduction sums the slots in a fixed order
+    if (lane == 0) a.gslot[(size_t)gw * NG + G] += v;
   });
 }
 
-// lambda_T = (u_T - target) on the interior, 0 on the halo; loss partial sums.
+// lambda_T = (u_T - target) on the interior, 0 on the halo; the loss partial
+// sum of every wave goes to its own slot (kSeedBlocks * 4 slots).
 template <typename T>
 __global__ __launch_bounds__(256) void adjoint_seed_kernel(const T* __restrict__ uT,
                                                            const T* __restrict__ target,
-                                                           T* __restrict__ lam, double* __restrict__ loss,
+                                                           T* __restrict__ lam, double* __restrict__ lslot,
                                                            int D0, int D2, long long s0, long long st,
                                                            int R) {
   const long long total = (long long)D0 * D2;
@@ -387,7 +399,7 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(const T* __restrict__
     part += 0.5 * (double)d * (double)d;
   }
   part = wave_sum<T>(part);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss, part);
+  if ((threadIdx.x & 63) == 0) lslot[blockIdx.x * 4 + (threadIdx.x >> 6)] = part;
 }
 
 // Copy the halo cells of `src` into `dst` (same layout), for the checkpoint

@@ -40,6 +40,15 @@ struct Variant {
 // Every stencil x dtype translation unit appends its variants here.
 std::vector<Variant>& registry();
 
+// The adjoint workspace pads every grid with kAdjPad cells on both ends of
+// each row (>= the widest tile overrun) and kAdjRowPad zero rows above and
+// below it (>= the deepest plane overrun of a segment), so the adjoint
+// kernel's loads need no bounds checks.
+constexpr int kAdjPad = 320;
+constexpr int kAdjRowPad = 64;
+// blocks of 256 threads of the adjoint seed kernel: one loss slot per wave
+constexpr int kSeedBlocks = 1024;
+
 // dtype-erased AdjArgs<T>
 struct AdjLaunch {
   const void* U;
@@ -48,14 +57,13 @@ struct AdjLaunch {
   const void* coef;
   double gscale;
   int has_div;
-  double* gacc;
+  double* gslot;  // NG doubles per wave tile
   int D0, D2;
   long long s0;
   int ntk, nseg, sl, SK;
-  int padded;
 };
 using AdjFn = hipError_t (*)(const AdjLaunch&, dim3 grid, dim3 block, hipStream_t);
-using SeedFn = hipError_t (*)(const void* uT, const void* target, void* lam, double* loss, int D0,
+using SeedFn = hipError_t (*)(const void* uT, const void* target, void* lam, double* lslot, int D0,
                               int D2, long long s0, long long st, int R, hipStream_t);
 using HaloFn = hipError_t (*)(const void* src, void* dst, int D0, int D2, long long s0, int R,
                               hipStream_t);

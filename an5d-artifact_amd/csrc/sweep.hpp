@@ -211,6 +211,18 @@ struct MemGet {
 template <int R>
 constexpr int default_unroll() { return R <= 2 ? 2 * R + 1 : 1; }
 
+// Input planes in flight ahead of their use.  A depth that divides the unroll
+// factor rotates by renaming; an unroll of 1 keeps a two-deep queue (one move
+// per cell and plane).  At most ~40 VGPRs of queued cells.
+template <int UNR, int CELLS32>
+constexpr int prefetch_depth() {
+  if (UNR == 1) return 2 * CELLS32 <= 40 ? 2 : 1;
+  int best = 1;
+  for (int d = 1; d <= UNR; d++)
+    if (UNR % d == 0 && d * CELLS32 <= 40) best = d;
+  return best;
+}
+
 // Rows a wave publishes per extended plane for the middle-dim exchange: its
 // first and last XJ rows when it holds at least 2*XJ rows (neighbours need
 // nothing else), otherwise all of them.
@@ -332,19 +344,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 
   Plane<T, CJ, CK> ring[BT][P];
 
+  // Every load reads real grid cells: plane, row and column indices are
+  // clamped into the grid (the stand-in values only reach cells outside it,
+  // which are never stored).  Loads are unconditional, so a queued plane never
+  // merges with an older value, which would wait for the load at once.
+  const bool wave_kfull = __builtin_amdgcn_ballot_w64(!kfull) == 0;
   auto load_plane = [&](Plane<T, CJ, CK>& pl, int p) {
+    p = p < 0 ? 0 : (p >= a.D0 ? a.D0 - 1 : p);
     const T* base = a.src + (long long)p * a.s0;
 #pragma unroll
     for (int jj = 0; jj < CJ; jj++) {
-      const int j = jw + jj;
-      if (DIM3 && (j < 0 || j >= a.D1)) continue;
-      const T* row = base + (DIM3 ? (long long)j * a.s1 : 0ll) + kl;
-      if (kfull) {
-        load_run<T, CK>(pl.v[jj], row);
+      int j = jw + jj;
+      if (DIM3) j = j < 0 ? 0 : (j >= a.D1 ? a.D1 - 1 : j);
+      const T* row = base + (DIM3 ? (long long)j * a.s1 : 0ll);
+      if (wave_kfull) {
+        load_run<T, CK>(pl.v[jj], row + kl);
       } else {
 #pragma unroll
-        for (int kk = 0; kk < CK; kk++)
-          if (kl + kk >= 0 && kl + kk < a.D2) pl.v[jj][kk] = row[kk];
+        for (int kk = 0; kk < CK; kk++) {
+          int k = kl + kk;
+          k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+          pl.v[jj][kk] = row[k];
+        }
       }
     }
   };
@@ -493,18 +514,22 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
     }
   };
 
-  // The next input plane is loaded one iteration ahead of its use.  The
-  // iteration count is padded to a multiple of UNR so the unrolled body has
-  // no exit inside it: the ring rotation then costs no register moves.  The
-  // padding iterations only produce planes past the segment, never stored.
-  Plane<T, CJ, CK> pre;
-  if (i0 - H >= 0) load_plane(pre, i0 - H);
+  // Input planes are loaded PF iterations ahead of their use.  The iteration
+  // count is padded to a multiple of UNR so the unrolled body has no exit
+  // inside it: the ring rotation then costs no register moves.  The padding
+  // iterations only produce planes past the segment, never stored.
+  constexpr int PF = prefetch_depth<UNR, CJ * CK * (int)(sizeof(T) / 4)>();
+  constexpr bool PF_ROT = UNR % PF == 0;  // queue slots rotate by renaming
+  Plane<T, CJ, CK> pre[PF];
+#pragma unroll
+  for (int f = 0; f < PF; f++) load_plane(pre[f], i0 - H + f);
   const int nit_pad = (nit + UNR - 1) / UNR * UNR;
   for (int n0 = 0; n0 < nit_pad; n0 += UNR) {
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
       const int n = n0 + u;
       const int p_in = i0 - H + n;
+      const int slot = PF_ROT ? u % PF : 0;
       if constexpr (STARX) {
         // Every stage's centre plane this iteration is slot R+1 of its input
         // ring before the rotation: publish their edge rows, one barrier,
@@ -539,8 +564,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       }
 #pragma unroll
       for (int p = 0; p < P - 1; p++) ring[0][p] = ring[0][p + 1];
-      ring[0][P - 1] = pre;
-      if (n + 1 < nit && p_in + 1 >= 0 && p_in + 1 < a.D0) load_plane(pre, p_in + 1);
+      ring[0][P - 1] = pre[slot];
+      if constexpr (!PF_ROT) {
+#pragma unroll
+        for (int f = 0; f + 1 < PF; f++) pre[f] = pre[f + 1];
+      }
+      load_plane(pre[PF_ROT ? slot : PF - 1], p_in + PF);
       static_for<1, BT + 1>([&](auto sc) {
         constexpr int S = decltype(sc)::value;
         Plane<T, CJ, CK> out;

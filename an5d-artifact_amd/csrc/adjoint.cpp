@@ -50,12 +50,16 @@ constexpr long long kRowPad = kAdjRowPad;
 struct AdjPlan {
   const Variant* fwd = nullptr;
   const AdjVariant* adj = nullptr;
-  int bt = 0, sl = 0;
-  std::vector<int> steps;
+  int bt = 0, sl = 0;    // adjoint block size and segment length
+  int btf = 0, fsl = 0;  // forward launches: steps per launch (a multiple of bt) and sl
+  std::vector<int> fsteps;  // forward launch schedule
+  std::vector<int> blocks;  // adjoint blocks in time order (<= bt steps each)
+  std::vector<int> fblock;  // first block of each forward launch
   Geom g;
   long long pitch = 0;  // elements per padded row
   size_t grid_bytes = 0, ws_bytes = 0;
-  // after the grids: per-wave gradient slots (nslots x ng doubles), then the
+  // after the NB+3 grids (checkpoints u at every block start and u_T, two
+  // lambda buffers): per-wave gradient slots (nslots x ng doubles), then the
   // seed kernel's per-wave loss slots (kSeedBlocks * 4 doubles)
   long long nslots = 0;
   int ng = 0;
@@ -64,6 +68,9 @@ struct AdjPlan {
 
 static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block);
 
+// The forward pass runs launches of btf = m*bt steps that also store the
+// states at every bt-th stage, so the checkpoints sit bt steps apart while the
+// forward sweep keeps its own (deeper) temporal block.
 static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& ap) {
   std::string err;
   if (!make_geom(p, ap.g, err)) return fail(AN5D_E_INVALID, err);
@@ -72,9 +79,12 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   ap.adj = adj_variant(p->stencil, p->dtype);
   if (!ap.adj) return fail(AN5D_E_UNSUPPORTED, "no adjoint kernels for this stencil/dtype");
   auto vs = variants_of(p->stencil, p->dtype);
-  int vi = cfg && cfg->variant >= 0 ? cfg->variant : 0;
-  if (vi >= (int)vs.size()) return fail(AN5D_E_INVALID, "variant index out of range");
-  ap.fwd = vs[vi];
+  if (cfg && cfg->variant >= (int)vs.size()) return fail(AN5D_E_INVALID, "variant index out of range");
+  // forward kernel: the caller's variant, else the tuned/default forward choice
+  an5d_config fc{0, cfg ? cfg->variant : -1, 0, AN5D_EXACT};
+  int bt_pref = 0, sl_pref = 0;
+  int rc = resolve(p, &fc, &ap.fwd, &bt_pref, &sl_pref);
+  if (rc) return rc;
   int bt = cfg && cfg->bt > 0 ? cfg->bt : ap.adj->btmax;
   if (bt > ap.adj->btmax || bt > ap.fwd->btmax) {
     if (cfg && cfg->bt > 0) return fail(AN5D_E_UNSUPPORTED, "bt exceeds the adjoint or forward btmax");
@@ -82,25 +92,37 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   }
   if (ap.fwd->kind == kDirect) bt = 1;
   ap.bt = bt;
-  int n = an5d_schedule(p->timesteps, bt, nullptr, 0);
-  ap.steps.resize(n);
-  an5d_schedule(p->timesteps, bt, ap.steps.data(), n);
+  int m = (bt_pref + bt / 2) / bt;
+  if (m < 1) m = 1;
+  while (m > 1 && m * bt > ap.fwd->btmax) m--;
+  ap.btf = m * bt;
+  ap.fsl = ap.btf == bt_pref ? sl_pref : 0;
+  // plain split (the checkpoints are separate buffers, so the ping-pong
+  // parity rule of the forward schedule does not apply)
+  ap.fsteps.assign(p->timesteps / ap.btf, ap.btf);
+  if (p->timesteps % ap.btf) ap.fsteps.push_back(p->timesteps % ap.btf);
+  ap.blocks.clear();
+  ap.fblock.clear();
+  for (int kf : ap.fsteps) {
+    ap.fblock.push_back((int)ap.blocks.size());
+    for (int t = 0; t < kf; t += bt) ap.blocks.push_back(kf - t < bt ? kf - t : bt);
+  }
   ap.sl = cfg ? cfg->sl : 0;
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   ap.pitch = (ap.g.D2 + 2 * kPad + 63) / 64 * 64;
   ap.grid_bytes = ((size_t)(ap.g.D0 + 2 * kRowPad) * ap.pitch * esz + 255) / 256 * 256;
-  // gradient slots: the most wave tiles any launch of the schedule uses
+  // gradient slots: the most wave tiles any adjoint launch uses
   ap.ng = stencil_rt(p->stencil).linear ? stencil_rt(p->stencil).ntaps : 2;
   ap.nslots = 0;
-  for (int k : ap.steps) {
+  for (int k : ap.blocks) {
     AdjLaunch l{};
     dim3 grid, block;
-    int rc = adj_launch_plan(ap, k, l, grid, block);
+    rc = adj_launch_plan(ap, k, l, grid, block);
     if (rc) return rc;
-    const long long n = (long long)l.ntk * l.nseg;
-    if (n > ap.nslots) ap.nslots = n;
+    const long long nt = (long long)l.ntk * l.nseg;
+    if (nt > ap.nslots) ap.nslots = nt;
   }
-  ap.gslot_off = ap.grid_bytes * (ap.steps.size() + 1 + 2);
+  ap.gslot_off = ap.grid_bytes * (ap.blocks.size() + 1 + 2);
   ap.lslot_off = ap.gslot_off + ((size_t)ap.nslots * ap.ng * sizeof(double) + 255) / 256 * 256;
   ap.ws_bytes = ap.lslot_off + (size_t)kSeedBlocks * 4 * sizeof(double);
   return AN5D_OK;
@@ -214,7 +236,7 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
   const void* dcoef = nullptr;
   rc = device_coefs(cbytes, &dcoef);
   if (rc) return rc;
-  const int L = (int)ap.steps.size();
+  const int L = (int)ap.blocks.size();  // checkpoint slot L holds u_T
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   char* base = static_cast<char*>(ws);
   auto ckpt = [&](int i) {
@@ -238,17 +260,28 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     e = ap.adj->halo(ckpt(0), ckpt(i), (int)g.D0, (int)g.D2, P, R, st);
     if (e != hipSuccess) return hip_fail(e, "halo copy");
   }
-  // forward with a checkpoint at every launch boundary
-  for (int i = 0; i < L; i++) {
+  // forward: launch j advances blocks fblock[j] .. fblock[j]+nb-1 and stores
+  // the state at every block boundary (the last one as its output)
+  const long long slot_elems = (long long)(ap.grid_bytes / esz);
+  for (size_t j = 0; j < ap.fsteps.size(); j++) {
+    const int kf = ap.fsteps[j];
+    const int b0 = ap.fblock[j];
+    const int nb = (kf + ap.bt - 1) / ap.bt;
     LaunchPlan lp;
-    if (!plan_launch(*ap.fwd, ap.steps[i], g, ap.sl, lp, err)) return fail(AN5D_E_UNSUPPORTED, err);
-    lp.args.src = ckpt(i);
-    lp.args.dst = ckpt(i + 1);
+    if (!plan_launch(*ap.fwd, kf, g, ap.fsl, lp, err)) return fail(AN5D_E_UNSUPPORTED, err);
+    lp.args.src = ckpt(b0);
+    lp.args.dst = ckpt(b0 + nb);
     lp.args.coef = dcoef;
     lp.args.gscale = gscale;
     lp.args.has_div = has_div;
     lp.args.s0 = P;
     lp.args.s1 = P;
+    unsigned mask = 0;
+    for (int t = 1; t < nb; t++) mask |= 1u << (t * ap.bt);
+    if (mask && ap.fwd->kind == kDirect) return fail(AN5D_E_UNSUPPORTED, "one-step kernel with intermediate outputs");
+    lp.args.mid = mask ? ckpt(b0 + 1) : nullptr;
+    lp.args.mid_stride = slot_elems;
+    lp.args.mid_mask = mask;
     e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
     if (e != hipSuccess) return hip_fail(e, "forward launch");
   }
@@ -265,7 +298,7 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
   // reverse blocks
   int cur = 0;
   for (int i = L - 1; i >= 0; i--) {
-    const int k = ap.steps[i];
+    const int k = ap.blocks[i];
     AdjLaunch l{};
     dim3 grid, block;
     rc = adj_launch_plan(ap, k, l, grid, block);

@@ -59,6 +59,9 @@ __device__ __forceinline__ void batch_tile(const BatchRec& r, int tile) {
   a.sl = r.sl;
   a.SK = r.SK;
   a.SJ = 1;
+  a.mid = nullptr;
+  a.mid_stride = 0;
+  a.mid_mask = 0;
   constexpr int UNR = default_unroll<Def<ID>::rad>();
   if constexpr (Def<ID>::divisor != 0) {
     if (r.has_div == 2) sweep_body<ID, float, BT, kBatchCK, 1, kBatchNW, true, 2, UNR>(a, tile);

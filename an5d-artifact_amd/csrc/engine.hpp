@@ -24,6 +24,9 @@ struct LaunchArgs {
   long long s0, s1;
   int ntk, ntj, nseg, sl;
   int SK, SJ;
+  void* mid;             // intermediate-stage outputs (see SweepArgs::mid_mask)
+  long long mid_stride;  // elements between consecutive intermediate outputs
+  unsigned mid_mask;     // bit S: store stage S; 0: none
 };
 
 using LaunchFn = hipError_t (*)(const LaunchArgs&, dim3 grid, dim3 block, size_t lds,

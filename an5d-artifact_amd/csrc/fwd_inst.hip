@@ -27,6 +27,9 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
   s.s0 = a.s0; s.s1 = a.s1;
   s.ntk = a.ntk; s.ntj = a.ntj; s.nseg = a.nseg; s.sl = a.sl;
   s.SK = a.SK; s.SJ = a.SJ;
+  s.mid = static_cast<U*>(a.mid);
+  s.mid_stride = a.mid_stride;
+  s.mid_mask = a.mid_mask;
   return s;
 }
 

@@ -40,6 +40,8 @@ std::vector<const Variant*> variants_of(int stencil, int dtype);
 bool make_geom(const an5d_problem* p, Geom& g, std::string& err);
 an5d_layout reference_layout(const Geom& g);
 an5d_layout padded_layout(const Geom& g);
+int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout, int* btout,
+            int* slout);
 bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp,
                  std::string& err, int mode = 0);
 bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,

@@ -45,6 +45,12 @@ struct SweepArgs {
   long long s0, s1;            // element strides of dim 0 and dim 1 (dim 2 is unit)
   int ntk, ntj, nseg, sl;      // tiles along dim 2 / dim 1, segments along dim 0, segment length
   int SK, SJ;                  // output cells per tile along dim 2 / dim 1
+  // Intermediate outputs (checkpoints for the adjoint): stage S < BT with bit
+  // S of mid_mask set also stores its owned cells into mid + m * mid_stride,
+  // m = the number of lower set bits.  mid_mask == 0: none.
+  T* __restrict__ mid;
+  long long mid_stride;
+  unsigned mid_mask;
 };
 
 // ---- cross-lane moves ------------------------------------------------------
@@ -370,8 +376,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
     }
   };
 
-  auto store_plane = [&](const Plane<T, CJ, CK>& pl, int q) {
-    T* base = a.dst + (long long)q * a.s0;
+  auto store_plane = [&](const Plane<T, CJ, CK>& pl, int q, T* dstbuf) {
+    T* base = dstbuf + (long long)q * a.s0;
 #pragma unroll
     for (int jj = 0; jj < CJ; jj++) {
       if (!jst[jj]) continue;
@@ -575,11 +581,18 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         Plane<T, CJ, CK> out;
         compute(ring[S - 1], out, p_in - S * R, sc);
         if constexpr (S < BT) {
+          // owned planes of stage S: n in [H + S*R, H + S*R + sle)
+          if ((a.mid_mask >> S) & 1u) {
+            if (n >= H + S * R && n < H + S * R + sle) {
+              const int m = __builtin_popcount(a.mid_mask & ((1u << S) - 1u));
+              store_plane(out, p_in - S * R, a.mid + (long long)m * a.mid_stride);
+            }
+          }
 #pragma unroll
           for (int p = 0; p < P - 1; p++) ring[S][p] = ring[S][p + 1];
           ring[S][P - 1] = out;
         } else {
-          if (n >= 2 * BT * R && n < nit) store_plane(out, p_in - BT * R);
+          if (n >= 2 * BT * R && n < nit) store_plane(out, p_in - BT * R, a.dst);
         }
       });
     }

@@ -10,7 +10,8 @@ using T = std::conditional_t<AN5D_F64, double, float>;
 constexpr int SID = AN5D_SID;
 using D = Def<SID>;
 
-// cells per lane and deepest block: keeps fr/ar rings under ~200 VGPRs
+// cells per lane and deepest block: keeps the kernels within 256 VGPRs (two
+// waves per SIMD)
 constexpr int adj_ck() {
   if (!D::linear) return 2;
   if (D::rad == 1 && !AN5D_F64) return 4;
@@ -18,8 +19,8 @@ constexpr int adj_ck() {
 }
 constexpr int adj_btmax() {
   if (!D::linear) return 2;
-  if (D::rad == 1) return D::ntaps > 5 ? 3 : 4;
-  if (D::rad == 2) return 2;
+  if (D::rad == 1) return D::ntaps > 5 ? 4 : 5;
+  if (D::rad == 2) return AN5D_F64 ? 2 : 4;  // deeper FP64 blocks exceed 256 VGPRs
   return 1;
 }
 

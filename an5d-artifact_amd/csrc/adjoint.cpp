@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void adj_finish_kernel(const double* __restric
 
 static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block) {
   const int R = ap.g.r, W = ap.adj->reach;
-  const long long HA = (2LL * k - 2) * R + W;
+  const long long ck = ap.adj->ck;
+  const long long HA = ((2LL * k - 2) * R + W + ck - 1) / ck * ck;  // whole lanes (adj_halo)
   const long long SK = 64LL * ap.adj->ck - 2 * HA;
   if (SK <= 0) return fail(AN5D_E_UNSUPPORTED, "adjoint block too deep for the tile");
   const long long nk = ap.g.D2 - 2 * R, n0 = ap.g.D0 - 2 * R;

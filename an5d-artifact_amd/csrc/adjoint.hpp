@@ -55,8 +55,11 @@ constexpr int adj_prefetch() {
 template <class D>
 constexpr int adj_reach() { return D::linear ? D::rad : 2; }
 
-template <int BT, int R, int W>
-constexpr int adj_halo() { return (2 * BT - 2) * R + W; }  // garbage spread along the fastest dim
+// Cells a tile gives up on each side along the fastest dim (the garbage
+// spread, rounded up to whole lanes so that every lane's CK cells are either
+// all owned or all not).
+template <int BT, int R, int W, int CK>
+constexpr int adj_halo() { return ((2 * BT - 2) * R + W + CK - 1) / CK * CK; }
 
 template <typename T, int CK, int XK>
 __device__ __forceinline__ void extend_row(const T (&src)[CK], T (&x)[CK + 2 * XK]) {
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   constexpr int W = adj_reach<D>();
   constexpr int DLT = W + (BT - 2) * R;  // adjoint lag behind the checkpoint input
   constexpr int HU = 2 * (BT - 1) * R + W;
-  constexpr int HA = adj_halo<BT, R, W>();
+  constexpr int HA = adj_halo<BT, R, W, CK>();
   constexpr int NT = D::linear ? D::ntaps : 1;
   constexpr int NG = D::linear ? D::ntaps : 2;  // gradient entries
   constexpr int XK = R;                          // fastest-dim extension of a plane
@@ -118,15 +121,20 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   const int sle = min(a.sl, a.D0 - R - i0);
   const int nit = sle + HU + DLT + BT * R;
 
-  bool kin[CK], kst[CK];
-  bool kst_all = true;
+  bool kin[CK];
+  bool lane_in = true;
 #pragma unroll
   for (int kk = 0; kk < CK; kk++) {
     const int k = kl + kk;
     kin[kk] = k >= R && k < a.D2 - R;
-    kst[kk] = k >= kv0 && k < kv1;
-    kst_all = kst_all && kst[kk];
+    lane_in = lane_in && kin[kk];
   }
+  // every cell of the wave interior along the fastest dim: no per-cell masks
+  const bool wave_in = __builtin_amdgcn_ballot_w64(!lane_in) == 0;
+  // the lane's cells belong to this tile (lane-aligned: HA and SK are whole
+  // lanes); in the last tile the owned lanes past the interior hold lambda = 0
+  const bool lown = kl >= kv0 && kl < kv0 + a.SK;
+  (void)kv1;
 
   T c[NT + 2];
 #pragma unroll
@@ -238,8 +246,12 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
           } else {
             val = eval_point<D, T>(get, c, div, has_div, gscale);
           }
-          out[KK] = (pin && kin[KK]) ? val : xin[R][KK + XK];
+          out[KK] = val;
         });
+        if (!(pin && wave_in)) {
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) out[kk] = (pin && kin[kk]) ? out[kk] : xin[R][kk + XK];
+        }
         shift(fr[M], out);
       });
       // ---- adjoint stages ---------------------------------------------------
@@ -250,9 +262,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
         const int qa = pU - DLT - A * R;
         const bool pin = qa >= R && qa < a.D0 - R;
         const bool own = qa >= i0 && qa < i0 + sle && n < nit;
-        // mu restricted to the cells this wave owns (0 elsewhere): the
-        // gradient sums run unconditionally instead of behind a branch
-        T mown[CK];
         // mu planes qa-R..qa+R, row-extended
         T mx[P][CK + 2 * XK];
         static_for<0, P>([&](auto pc) {
@@ -270,7 +279,7 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
               constexpr TapDef tp = D::taps[Q];
               acc = fma_t(ct[Q], mx[R - tp.d0][KK - tp.d2 + XK], acc);
             });
-            out[KK] = (pin && kin[KK]) ? acc : T(0);
+            out[KK] = acc;
           });
           {
             // sum_x mu(x) u_t(x + d_q) over owned cells of plane qa
@@ -279,15 +288,18 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
               constexpr int PP = decltype(pc)::value;
               extend_row<T, CK, XK>(fr[M][BM + (W - R) + PP], ux[PP]);
             });
-            static_for<0, CK>([&](auto kc) {
-              constexpr int KK = decltype(kc)::value;
-              mown[KK] = (own && kst[KK]) ? mx[R][KK + XK] : T(0);
-              static_for<0, NT>([&](auto qc) {
-                constexpr int Q = decltype(qc)::value;
-                constexpr TapDef tp = D::taps[Q];
-                gpart[Q] = fma_t(mown[KK], ux[R + tp.d0][KK + tp.d2 + XK], gpart[Q]);
+            // mu is 0 off the interior; lanes this tile does not own are
+            // dropped when the partial sums are flushed
+            if (own) {
+              static_for<0, CK>([&](auto kc) {
+                constexpr int KK = decltype(kc)::value;
+                static_for<0, NT>([&](auto qc) {
+                  constexpr int Q = decltype(qc)::value;
+                  constexpr TapDef tp = D::taps[Q];
+                  gpart[Q] = fma_t(mx[R][KK + XK], ux[R + tp.d0][KK + tp.d2 + XK], gpart[Q]);
+                });
               });
-            });
+            }
           }
         } else {
           // gradient2d: F(x) = C + scale * rs(x), rs = (eps + sum_m (C - N_m)^2)^-1/2
@@ -331,9 +343,10 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
               gC[KK] = mu * (T(1) + f * (dn + ds + de + dw));
               gE[KK] = mu * (-f * de);  // to x + (0,1)
               gW[KK] = mu * (-f * dw);  // to x - (0,1)
-              mown[KK] = (own && kst[KK]) ? mu : T(0);
-              gpart[0] = gpart[0] + mown[KK] * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
-              gpart[1] = gpart[1] + mown[KK] * rs;           // d/d scale
+              if (own) {
+                gpart[0] = gpart[0] + mu * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
+                gpart[1] = gpart[1] + mu * rs;           // d/d scale
+              }
             }
           });
           T gEx[CK + 2], gWx[CK + 2];
@@ -342,32 +355,30 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
           static_for<0, CK>([&](auto kc) {
             constexpr int KK = decltype(kc)::value;
             const T v = gC[KK] + fN[KK] + fS[KK] + gEx[KK] + gWx[KK + 2];
-            out[KK] = (pin && kin[KK]) ? v : T(0);
+            out[KK] = v;
           });
+        }
+        if (!(pin && wave_in)) {
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) out[kk] = (pin && kin[kk]) ? out[kk] : T(0);
         }
         if constexpr (A < BT) {
           shift(ar[A], out);
         } else {
-          if (own) {
-            T* row = a.Lout + (long long)qa * a.s0 + kl;
-            if (kst_all) {
-              store_run<T, CK>(row, out);
-            } else {
-#pragma unroll
-              for (int kk = 0; kk < CK; kk++)
-                if (kst[kk]) row[kk] = out[kk];
-            }
-          }
+          if (own && lown) store_run<T, CK>(a.Lout + (long long)qa * a.s0 + kl, out);
         }
       });
     }
     // flush the per-lane partial sums to FP64 once per unrolled body
 #pragma unroll
-    for (int g = 0; g < NG; g++) { gsum[g] += (double)gpart[g]; gpart[g] = 0; }
+    for (int g = 0; g < NG; g++) {
+      gsum[g] += lown ? (double)gpart[g] : 0.0;
+      gpart[g] = 0;
+    }
   }
   static_for<0, NG>([&](auto gc) {
     constexpr int G = decltype(gc)::value;
-    double v = wave_sum<T>(gsum[G] + (double)gpart[G]);
+    double v = wave_sum<T>(gsum[G] + (lown ? (double)gpart[G] : 0.0));
     if constexpr (D::linear) {
       // d u_{t+1} / d c_q = s_q u_t(x + d_q) / div
       v = (D::taps[G].sign > 0 ? v : -v) / (has_div ? (double)div : 1.0);

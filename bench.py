@@ -112,7 +112,7 @@ def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3):
     gc = torch.zeros(d["ntaps"] if d["linear"] else 2, dtype=torch.float64, device="cuda")
     loss = torch.zeros(1, dtype=torch.float64, device="cuda")
     best_fa = None
-    for abt in (2, 3, 4):
+    for abt in (2, 3, 4, 5):
         if abt > 1 and not d["linear"] and abt > 2:
             continue
         for sl in (0, 128, 256):

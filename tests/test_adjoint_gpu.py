@@ -24,16 +24,23 @@ def problem(name, n, dt, seed):
     return u0, target, np.array([1e-4, 0.01]), -1.0
 
 
+def adj_btmax(name, dt):
+    """Deepest compiled adjoint block (mirrors the kernel registry)."""
+    d = O.info(name)
+    if not d["linear"]:
+        return 2
+    if d["rad"] == 1:
+        return 4 if d["ntaps"] > 5 else 5
+    if d["rad"] == 2:
+        return 2 if dt == np.float64 else 4
+    return 1
+
+
 def cases():
     out = []
     for name in NAMES_2D:
         for dt in (np.float32, np.float64):
-            adj_bt = {1: 4, 2: 2}.get(O.info(name)["rad"], 1)
-            if not O.info(name)["linear"]:
-                adj_bt = 2
-            elif O.info(name)["rad"] == 1 and O.info(name)["ntaps"] > 5:
-                adj_bt = 3
-            for bt in sorted({1, adj_bt}):
+            for bt in range(1, adj_btmax(name, dt) + 1):
                 out.append((name, dt, bt))
     return out
 

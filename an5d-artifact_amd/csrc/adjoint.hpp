@@ -4,8 +4,10 @@
 // L = 1/2 sum_interior (u_T - target)^2.  Reverse mode over one time block
 // [t0, t0+BT): given the forward checkpoint u_{t0} and lambda_{t0+BT}
 // (= dL/du_{t0+BT} on the interior, zero on the halo), one launch
-//   * recomputes u_{t0+1..t0+BT-1} from the checkpoint (forward stages, the
-//     exact forward expression, so the states equal the forward pass),
+//   * recomputes u_{t0+1..t0+BT-1} from the checkpoint (forward stages in
+//     working precision: folded coefficients and FMAs, gradient2d with the
+//     hardware reciprocal square root in FP32; the states match the forward
+//     pass to rounding, which only perturbs the Jacobians),
 //   * runs BT adjoint stages lambda_t = J_t^T mu_{t+1}, mu = lambda masked to
 //     the interior:  linear stencils: lambda_t(y) = sum_q s_q c_q/div *
 //     mu_{t+1}(y - d_q) (the transposed taps);  gradient2d: the Jacobian of
@@ -18,7 +20,7 @@
 // forward state u_{t0+BT-a} it needs (W planes either side) still in the
 // stage's ring.  Fastest-dim neighbours come from DPP as in the forward.
 // Tolerance-based (FP32 vs an FP64 oracle): adjoint arithmetic contracts to
-// FMA; the recomputed forward states are exact.
+// FMA; the checkpoints themselves come from the exact forward sweep.
 #pragma once
 
 #include "engine.hpp"
@@ -54,6 +56,9 @@ constexpr int adj_prefetch() {
 
 template <class D>
 constexpr int adj_reach() { return D::linear ? D::rad : 2; }
+
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / __builtin_sqrt(x); }
 
 // Cells a tile gives up on each side along the fastest dim (the garbage
 // spread, rounded up to whole lanes so that every lane's CK cells are either
@@ -174,6 +179,13 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
       }
   double gsum[NG];
   T gpart[NG];
+  // gradient2d: per adjoint stage, the partial lambda of the plane above the
+  // one being finished and the S fluxes of its plane
+  T nl_acc[BT][CK], nl_cs[BT][CK];
+#pragma unroll
+  for (int m = 0; m < BT; m++)
+#pragma unroll
+    for (int kk = 0; kk < CK; kk++) nl_acc[m][kk] = nl_cs[m][kk] = T(0);
 #pragma unroll
   for (int g = 0; g < NG; g++) { gsum[g] = 0.0; gpart[g] = 0; }
 
@@ -200,8 +212,8 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   T preU[PF][CK], preL[PF][CK];
 #pragma unroll
   for (int f = 0; f < PF; f++) {
-    load_run<T, CK>(preU[f], up + koff);
-    load_run<T, CK>(preL[f], lp + koff);
+    gload_run<false, T, CK>(preU[f], up + koff);
+    gload_run<false, T, CK>(preL[f], lp + koff);
     up += s0;
     lp += s0;
   }
@@ -215,8 +227,8 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
       shift(ar[0], preL[u % PF]);
       // unconditional (the padding rows absorb the overrun): a conditional
       // load would merge with the old slot value and wait for the load at once
-      load_run<T, CK>(preU[u % PF], up + koff);
-      load_run<T, CK>(preL[u % PF], lp + koff);
+      gload_run<false, T, CK>(preU[u % PF], up + koff);
+      gload_run<false, T, CK>(preL[u % PF], lp + koff);
       up += s0;
       lp += s0;
       // ---- forward recompute (stages 1..BT-1) ------------------------------
@@ -244,7 +256,13 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
               val = fma_t(ct[Q], xin[R + tp.d0][KK + tp.d2 + XK], val);
             });
           } else {
-            val = eval_point<D, T>(get, c, div, has_div, gscale);
+            // gradient2d in working precision: the states feed only the
+            // Jacobians, which the tolerance covers
+            const T cc = xin[R][KK + XK];
+            const T dn = cc - xin[R - 1][KK + XK], ds = cc - xin[R + 1][KK + XK];
+            const T de = cc - xin[R][KK + XK + 1], dw = cc - xin[R][KK + XK - 1];
+            val = cc + (T)gscale * rsqrt_t(c[0] + dn * dn + ds * ds + de * de + dw * dw);
+            (void)get;
           }
           out[KK] = val;
         });
@@ -304,58 +322,49 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
         } else {
           // gradient2d: F(x) = C + scale * rs(x), rs = (eps + sum_m (C - N_m)^2)^-1/2
           //   dF/dC = 1 + f * sum_m (C - N_m), dF/dN_m = -f (C - N_m), f = -scale rs^3
-          // lambda(y) = mu(y) dF/dC(y) + sum_m mu(y - e_m) dF/dN_m(y - e_m)
+          // Each x of plane qa+1 (the newest mu plane) sends mu(x) dF(x)/du(y)
+          // to itself and its four neighbours y, from one Jacobian evaluation:
+          //   lambda(qa) = [own terms of plane qa, its E/W fluxes and the S flux
+          //                 of plane qa-1: kept from the previous iteration]
+          //              + N flux of plane qa+1
 #pragma clang fp contract(fast)
-          T ux[5][CK + 2 * XU];
-          static_for<0, 5>([&](auto pc) {
+          T ux[3][CK + 2 * XU];  // u_t planes qa, qa+1, qa+2
+          static_for<0, 3>([&](auto pc) {
             constexpr int PP = decltype(pc)::value;
-            extend_row<T, CK, XU>(fr[M][BM + PP], ux[PP]);
+            extend_row<T, CK, XU>(fr[M][BM + 2 + PP], ux[PP]);
           });
           const T eps = c[0];
           const T sc = (T)gscale;
-          T fN[CK], fS[CK], gE[CK], gW[CK], gC[CK];
+          const bool own1 = qa + 1 >= i0 && qa + 1 < i0 + sle;
+          T cN[CK], cS[CK], cE[CK], cW[CK], cC[CK];
           static_for<0, CK>([&](auto kc) {
             constexpr int KK = decltype(kc)::value;
             constexpr int K = KK + XU;
-            {  // x on plane qa+1: its N neighbour is y
-              const T cc = ux[3][K];
-              const T dn = cc - ux[2][K], ds = cc - ux[4][K], de = cc - ux[3][K + 1], dw = cc - ux[3][K - 1];
-              const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
-              const T rs = T(1) / __builtin_sqrt(s2);
-              const T f = -sc * rs * rs * rs;
-              fN[KK] = mx[R + 1][KK + XK] * (-f * dn);
-            }
-            {  // x on plane qa-1: its S neighbour is y
-              const T cc = ux[1][K];
-              const T dn = cc - ux[0][K], ds = cc - ux[2][K], de = cc - ux[1][K + 1], dw = cc - ux[1][K - 1];
-              const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
-              const T rs = T(1) / __builtin_sqrt(s2);
-              const T f = -sc * rs * rs * rs;
-              fS[KK] = mx[R - 1][KK + XK] * (-f * ds);
-            }
-            {  // x on plane qa: own centre term and the E/W fluxes
-              const T cc = ux[2][K];
-              const T dn = cc - ux[1][K], ds = cc - ux[3][K], de = cc - ux[2][K + 1], dw = cc - ux[2][K - 1];
-              const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
-              const T rs = T(1) / __builtin_sqrt(s2);
-              const T f = -sc * rs * rs * rs;
-              const T mu = mx[R][KK + XK];
-              gC[KK] = mu * (T(1) + f * (dn + ds + de + dw));
-              gE[KK] = mu * (-f * de);  // to x + (0,1)
-              gW[KK] = mu * (-f * dw);  // to x - (0,1)
-              if (own) {
-                gpart[0] = gpart[0] + mu * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
-                gpart[1] = gpart[1] + mu * rs;           // d/d scale
-              }
+            const T cc = ux[1][K];
+            const T dn = cc - ux[0][K], ds = cc - ux[2][K], de = cc - ux[1][K + 1], dw = cc - ux[1][K - 1];
+            const T s2 = eps + dn * dn + ds * ds + de * de + dw * dw;
+            const T rs = rsqrt_t(s2);
+            const T f = -sc * rs * rs * rs;
+            const T mu = mx[R + 1][KK + XK];
+            const T mf = -mu * f;
+            cN[KK] = mf * dn;  // to the plane above
+            cS[KK] = mf * ds;  // to the plane below
+            cE[KK] = mf * de;  // to x + (0,1)
+            cW[KK] = mf * dw;  // to x - (0,1)
+            cC[KK] = mu - mf * (dn + ds + de + dw);
+            if (own1) {
+              gpart[0] = gpart[0] + mu * (T(0.5) * f);  // d/d eps = -1/2 scale rs^3 = f/2
+              gpart[1] = gpart[1] + mu * rs;           // d/d scale
             }
           });
-          T gEx[CK + 2], gWx[CK + 2];
-          extend_row<T, CK, 1>(gE, gEx);
-          extend_row<T, CK, 1>(gW, gWx);
+          T cEx[CK + 2], cWx[CK + 2];
+          extend_row<T, CK, 1>(cE, cEx);
+          extend_row<T, CK, 1>(cW, cWx);
           static_for<0, CK>([&](auto kc) {
             constexpr int KK = decltype(kc)::value;
-            const T v = gC[KK] + fN[KK] + fS[KK] + gEx[KK] + gWx[KK + 2];
-            out[KK] = v;
+            out[KK] = nl_acc[A - 1][KK] + cN[KK];
+            nl_acc[A - 1][KK] = cC[KK] + cEx[KK] + cWx[KK + 2] + nl_cs[A - 1][KK];
+            nl_cs[A - 1][KK] = cS[KK];
           });
         }
         if (!(pin && wave_in)) {
@@ -365,7 +374,7 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
         if constexpr (A < BT) {
           shift(ar[A], out);
         } else {
-          if (own && lown) store_run<T, CK>(a.Lout + (long long)qa * a.s0 + kl, out);
+          if (own && lown) gstore_run<false, T, CK>(a.Lout + (long long)qa * a.s0 + kl, out);
         }
       });
     }

@@ -99,6 +99,53 @@ __device__ __forceinline__ void store_run(T* p, const T (&src)[N]) {
   __builtin_memcpy(p, &t, sizeof(t));
 }
 
+// Global accesses of 4-byte-aligned runs; NT: non-temporal (streaming) load
+// or store.  The non-temporal hint pays where a launch is bandwidth-bound
+// (one or two fused steps) and costs where it is compute-bound (the halo
+// re-reads of overlapping tiles then miss the cache).
+template <typename T, int N>
+struct NtVec;
+template <> struct NtVec<float, 1> { typedef float type __attribute__((aligned(4))); };
+template <> struct NtVec<float, 2> { typedef float type __attribute__((ext_vector_type(2), aligned(4))); };
+template <> struct NtVec<float, 4> { typedef float type __attribute__((ext_vector_type(4), aligned(4))); };
+template <> struct NtVec<float, 8> { typedef float type __attribute__((ext_vector_type(8), aligned(4))); };
+template <> struct NtVec<double, 1> { typedef double type __attribute__((aligned(4))); };
+template <> struct NtVec<double, 2> { typedef double type __attribute__((ext_vector_type(2), aligned(4))); };
+template <> struct NtVec<double, 4> { typedef double type __attribute__((ext_vector_type(4), aligned(4))); };
+template <> struct NtVec<double, 8> { typedef double type __attribute__((ext_vector_type(8), aligned(4))); };
+
+template <bool NT, typename T, int N>
+__device__ __forceinline__ void gload_run(T (&dst)[N], const T* p) {
+  if constexpr (NT) {
+    using V = typename NtVec<T, N>::type;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    if constexpr (N == 1) {
+      dst[0] = v;
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; i++) dst[i] = v[i];
+    }
+  } else {
+    load_run<T, N>(dst, p);
+  }
+}
+template <bool NT, typename T, int N>
+__device__ __forceinline__ void gstore_run(T* p, const T (&src)[N]) {
+  if constexpr (NT) {
+    using V = typename NtVec<T, N>::type;
+    V v;
+    if constexpr (N == 1) {
+      v = src[0];
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; i++) v[i] = src[i];
+    }
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+  } else {
+    store_run<T, N>(p, src);
+  }
+}
+
 // ---- correctly rounded division by a constant --------------------------------
 // has_div == 2: q0 = x*rc, e' = fma(q0, d, -x) (exact residual, negated),
 // q = fma(-e', rc, q0) with rc = RN(1/d) (Markstein's correction step; the
@@ -280,6 +327,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   constexpr int XJ = MID ? R : 0;
   constexpr int XK = FAST ? R : 0;
   constexpr int H = BT * R;
+  constexpr bool STREAM = BT <= 2;  // non-temporal grid loads and stores
   static_assert(!FAST || CK >= R, "a lane must hold at least R cells");
   static_assert(DIM3 || CJ == 1, "2D sweeps hold one row per lane");
   static_assert(!MID || NW * CJ > 2 * H, "tile too thin for BT");
@@ -364,7 +412,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       if (DIM3) j = j < 0 ? 0 : (j >= a.D1 ? a.D1 - 1 : j);
       const T* row = base + (DIM3 ? (long long)j * a.s1 : 0ll);
       if (wave_kfull) {
-        load_run<T, CK>(pl.v[jj], row + kl);
+        gload_run<STREAM, T, CK>(pl.v[jj], row + kl);
       } else {
 #pragma unroll
         for (int kk = 0; kk < CK; kk++) {
@@ -383,7 +431,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       if (!jst[jj]) continue;
       T* row = base + (DIM3 ? (long long)(jw + jj) * a.s1 : 0ll) + kl;
       if (kst_all) {
-        store_run<T, CK>(row, pl.v[jj]);
+        gstore_run<STREAM, T, CK>(row, pl.v[jj]);
       } else {
 #pragma unroll
         for (int kk = 0; kk < CK; kk++)

@@ -14,13 +14,13 @@ using D = Def<SID>;
 // waves per SIMD)
 constexpr int adj_ck() {
   if (!D::linear) return 2;
-  if (D::rad == 1 && !AN5D_F64) return 4;
+  if (D::rad <= 2 && !AN5D_F64) return 4;
   return D::rad <= 2 ? 2 : 4;  // a lane holds at least R cells
 }
 constexpr int adj_btmax() {
   if (!D::linear) return 2;
   if (D::rad == 1) return D::ntaps > 5 ? 4 : 5;
-  if (D::rad == 2) return AN5D_F64 ? 2 : 4;  // deeper FP64 blocks exceed 256 VGPRs
+  if (D::rad == 2) return AN5D_F64 ? 2 : 3;  // deeper blocks exceed 256 VGPRs
   return 1;
 }
 
@@ -40,6 +40,23 @@ hipError_t launch_adj(const AdjLaunch& l, dim3 grid, dim3 block, hipStream_t st)
   return hipGetLastError();
 }
 
+template <int BT>
+int occupancy_of() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, reinterpret_cast<const void*>(&adjoint_kernel<SID, T, BT, adj_ck(), 4>), 64 * 4, 0) !=
+      hipSuccess)
+    return 0;
+  return n;
+}
+int occupancy(int bt) {
+  int n = 0;
+  static_for<1, adj_btmax() + 1>([&](auto bc) {
+    constexpr int BT = decltype(bc)::value;
+    if (bt == BT) n = occupancy_of<BT>();
+  });
+  return n;
+}
 hipError_t launch_seed(const void* uT, const void* target, void* lam, double* lslot, int D0, int D2,
                        long long s0, long long stgt, int R, hipStream_t st) {
   hipLaunchKernelGGL((adjoint_seed_kernel<T>), dim3(kSeedBlocks), dim3(256), 0, st, static_cast<const T*>(uT),
@@ -71,6 +88,7 @@ void AN5D_CAT(register_adjoint_, AN5D_SID, AN5D_F64)(std::vector<AdjVariant>& ou
   v.reach = adj_reach<D>();
   v.seed = &launch_seed;
   v.halo = &launch_halo;
+  v.occupancy = &occupancy;
   static_for<1, adj_btmax() + 1>([&](auto bc) {
     constexpr int BT = decltype(bc)::value;
     v.fn[BT] = &launch_adj<BT>;

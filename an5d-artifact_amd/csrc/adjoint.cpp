@@ -66,7 +66,8 @@ struct AdjPlan {
   size_t gslot_off = 0, lslot_off = 0;
 };
 
-static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block);
+static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block,
+                           bool sizing = false);
 
 // The forward pass runs launches of btf = m*bt steps that also store the
 // states at every bt-th stage, so the checkpoints sit bt steps apart while the
@@ -94,7 +95,7 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   ap.bt = bt;
   int m = (bt_pref + bt / 2) / bt;
   if (m < 1) m = 1;
-  while (m > 1 && m * bt > ap.fwd->btmax) m--;
+  while (m > 1 && (m * bt > ap.fwd->btmax || m * bt > kMaxMidBT)) m--;
   ap.btf = m * bt;
   ap.fsl = ap.btf == bt_pref ? sl_pref : 0;
   // plain split (the checkpoints are separate buffers, so the ping-pong
@@ -117,7 +118,7 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   for (int k : ap.blocks) {
     AdjLaunch l{};
     dim3 grid, block;
-    rc = adj_launch_plan(ap, k, l, grid, block);
+    rc = adj_launch_plan(ap, k, l, grid, block, true);
     if (rc) return rc;
     const long long nt = (long long)l.ntk * l.nseg;
     if (nt > ap.nslots) ap.nslots = nt;
@@ -164,7 +165,13 @@ __global__ __launch_bounds__(256) void adj_finish_kernel(const double* __restric
   }
 }
 
-static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block) {
+// Segments along dim 0: by default as many as fill the GPU's resident wave
+// slots in one round (a second, partly filled round would leave most SIMDs
+// idle), at least kMinSl planes each.  `sizing` gives the most segments any
+// launch can use (the gradient-slot count) without touching the device.
+constexpr long long kMinSl = 16;
+static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block,
+                           bool sizing) {
   const int R = ap.g.r, W = ap.adj->reach;
   const long long ck = ap.adj->ck;
   const long long HA = ((2LL * k - 2) * R + W + ck - 1) / ck * ck;  // whole lanes (adj_halo)
@@ -174,10 +181,20 @@ static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, d
   const long long ntk = cdiv(nk, SK);
   long long sl = ap.sl;
   if (sl <= 0) {
-    const long long warm = (4LL * k - 4) * R + 2 * W;
-    sl = cdiv(n0, cdiv(4096, ntk));
-    if (sl < 4 * warm) sl = 4 * warm;
-    if (sl < 16) sl = 16;
+    long long slots = 0;
+    if (!sizing) {
+      const int occ = ap.adj->occupancy ? ap.adj->occupancy(k) : 0;
+      slots = (long long)occ * device_cus() * ap.adj->nw;
+    }
+    if (sizing) {
+      sl = kMinSl;
+    } else if (slots > 0) {
+      const long long nseg = slots / ntk > 1 ? slots / ntk : 1;
+      sl = cdiv(n0, nseg);
+    } else {
+      sl = cdiv(n0, cdiv(4096, ntk));
+    }
+    if (sl < kMinSl) sl = kMinSl;
   }
   if (sl > n0) sl = n0;
   const long long nseg = cdiv(n0, sl);
@@ -279,7 +296,10 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     lp.args.s1 = P;
     unsigned mask = 0;
     for (int t = 1; t < nb; t++) mask |= 1u << (t * ap.bt);
-    if (mask && ap.fwd->kind == kDirect) return fail(AN5D_E_UNSUPPORTED, "one-step kernel with intermediate outputs");
+    if (mask) {
+      lp.fn = ap.fwd->fn_mid[kf];
+      if (!lp.fn) return fail(AN5D_E_UNSUPPORTED, "no forward kernel with intermediate outputs");
+    }
     lp.args.mid = mask ? ckpt(b0 + 1) : nullptr;
     lp.args.mid_stride = slot_elems;
     lp.args.mid_mask = mask;

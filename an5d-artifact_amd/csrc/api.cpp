@@ -117,10 +117,33 @@ an5d_layout padded_layout(const Geom& g) {
 // ---- launch planning ----------------------------------------------------------
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
-int default_sl(const Variant& v, int bt, const Geom& g, long long ntk, long long ntj) {
+int device_cus() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return n;
+}
+
+// Default segment length along dim 0: as many segments as fill the GPU's
+// resident workgroup slots in one round (a partly filled second round would
+// leave most SIMDs idle); without a device to ask, about 8192 wave tiles.
+int default_sl(const Variant& v, int bt, const Geom& g, long long ntk, long long ntj, int mode) {
   const long long n0 = g.D0 - 2 * g.r;
-  long long tiles = ntk * ntj * (v.kind == kSweep && g.dim == 3 ? v.nw : 1);
-  long long nseg = cdiv(8192, tiles > 0 ? tiles : 1);
+  const bool tiles3 = v.kind == kSweep && g.dim == 3;
+  long long slots = 0;
+  if (v.kind == kSweep && bt >= 1 && bt <= kMaxBT && v.occ[bt]) {
+    const int occ = v.occ[bt](mode);
+    const int cus = occ > 0 ? device_cus() : 0;
+    slots = (long long)occ * cus * (tiles3 ? 1 : v.nw);  // workgroups (3D) or waves (2D)
+  }
+  long long nseg;
+  if (slots > 0) {
+    const long long per_seg = tiles3 ? ntk * ntj : ntk;
+    nseg = slots / per_seg > 1 ? slots / per_seg : 1;
+  } else {
+    long long tiles = ntk * ntj * (tiles3 ? v.nw : 1);
+    nseg = cdiv(8192, tiles > 0 ? tiles : 1);
+  }
   long long sl = cdiv(n0, nseg);
   long long minsl = 8LL * bt * g.r;
   if (sl < minsl) sl = minsl;
@@ -155,7 +178,7 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
     if (SJ <= 0) { err = "bt too deep for this variant's tile height"; return false; }
     ntj = cdiv(nj, SJ);
   }
-  if (sl <= 0) sl = default_sl(v, bt, g, ntk, ntj);
+  if (sl <= 0) sl = default_sl(v, bt, g, ntk, ntj, mode);
   const long long nseg = cdiv(n0, sl);
   a.ntk = (int)ntk; a.ntj = (int)ntj; a.nseg = (int)nseg; a.sl = sl;
   a.SK = (int)SK; a.SJ = (int)SJ;

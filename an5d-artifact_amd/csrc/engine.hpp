@@ -9,6 +9,7 @@
 namespace an5d {
 
 constexpr int kMaxBT = 16;
+constexpr int kMaxMidBT = 8;  // deepest forward block with intermediate-stage outputs
 
 enum KernelKind { kSweep = 0, kDirect = 1 };
 enum DType { kF32 = 0, kF64 = 1 };
@@ -38,6 +39,8 @@ struct Variant {
   size_t lds[kMaxBT + 1];  // dynamic LDS bytes per workgroup, by bt
   LaunchFn fn[kMaxBT + 1];       // fn[bt], bt in [1, btmax], exact mode
   LaunchFn fn_fast[kMaxBT + 1];  // fast (FMA) mode, same shapes
+  int (*occ[kMaxBT + 1])(int mode);  // resident workgroups per CU of fn[bt] / fn_fast[bt]
+  LaunchFn fn_mid[kMaxBT + 1];  // exact mode storing intermediate stages (2D, bt <= kMaxMidBT)
 };
 
 // Every stencil x dtype translation unit appends its variants here.
@@ -77,6 +80,7 @@ struct AdjVariant {
   SeedFn seed;
   HaloFn halo;
   AdjFn fn[kMaxBT + 1];
+  int (*occupancy)(int bt);  // resident workgroups per CU of fn[bt] (0: unknown)
 };
 
 std::vector<AdjVariant>& adj_registry();

@@ -33,18 +33,38 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
   return s;
 }
 
-template <int BT, int CK, int CJ, int NW, bool EXACT>
+template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS = false>
 hipError_t launch_sweep(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
   if (lds > 65536) {
     // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default once
     static const hipError_t attr = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>),
+        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (attr != hipSuccess) return attr;
   }
-  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>), grid, block, lds, st,
+  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>), grid, block, lds, st,
                      to_args<T>(a));
   return hipGetLastError();
+}
+
+// Resident workgroups per CU (0 if the query fails); computed once.
+template <int BT, int CK, int CJ, int NW, bool EXACT>
+int occupancy_once() {
+  static const int n = [] {
+    const void* k = reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>);
+    const size_t lds = sweep_lds_bytes<SID, T, CK, CJ, NW>(BT);
+    if (lds > 65536 &&
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * NW, lds) != hipSuccess) return 0;
+    return blocks;
+  }();
+  return n;
+}
+template <int BT, int CK, int CJ, int NW>
+int occupancy_sweep(int mode) {
+  return mode ? occupancy_once<BT, CK, CJ, NW, false>() : occupancy_once<BT, CK, CJ, NW, true>();
 }
 
 template <int CK>
@@ -67,6 +87,10 @@ void add_sweep(std::vector<Variant>& out) {
       v.lds[BT] = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>(BT);
       v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true>;
       v.fn_fast[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, false>;
+      v.occ[BT] = &occupancy_sweep<BT, sh.ck, sh.cj, sh.nw>;
+      // intermediate-stage outputs: the adjoint's checkpointing forward pass (2D, exact)
+      if constexpr (Def<SID>::dim == 2 && BT >= 2 && BT <= kMaxMidBT)
+        v.fn_mid[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true, true>;
     });
     out.push_back(v);
   }

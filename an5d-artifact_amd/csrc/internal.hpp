@@ -51,6 +51,7 @@ int device_grid(size_t bytes, void** out);
 int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
                  const an5d_config* cfg, int steps_only, int c0_first, hipStream_t st);
 double sb_time_ms();
+int device_cus();  // compute units of the current device (0 if unknown)
 
 // autotuner table (autotune.cpp)
 bool tuned_lookup(const an5d_problem* p, int mode, TunedChoice* out);

@@ -313,7 +313,10 @@ constexpr size_t sweep_lds_bytes(int bt) {
 
 // tile >= 0 (2D only): process that wave tile instead of the one implied by
 // blockIdx (used by the batched work-queue kernel).
-template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT, int DM, int UNR>
+// MIDS: the kernel also stores intermediate stages (SweepArgs::mid_mask);
+// compiled only where used, since the extra store paths enlarge every stage.
+template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT, int DM, int UNR,
+          bool MIDS = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1) {
   using D = Def<ID>;
   constexpr int R = D::rad;
@@ -630,10 +633,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         compute(ring[S - 1], out, p_in - S * R, sc);
         if constexpr (S < BT) {
           // owned planes of stage S: n in [H + S*R, H + S*R + sle)
-          if ((a.mid_mask >> S) & 1u) {
-            if (n >= H + S * R && n < H + S * R + sle) {
-              const int m = __builtin_popcount(a.mid_mask & ((1u << S) - 1u));
-              store_plane(out, p_in - S * R, a.mid + (long long)m * a.mid_stride);
+          if constexpr (MIDS) {
+            if ((a.mid_mask >> S) & 1u) {
+              if (n >= H + S * R && n < H + S * R + sle) {
+                const int m = __builtin_popcount(a.mid_mask & ((1u << S) - 1u));
+                store_plane(out, p_in - S * R, a.mid + (long long)m * a.mid_stride);
+              }
             }
           }
 #pragma unroll
@@ -648,18 +653,18 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 }
 
 // The division mode is fixed per launch: dispatch once, not per cell.
-template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT = true,
+template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT = true, bool MIDS = false,
           int UNR = default_unroll<Def<ID>::rad>()>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(SweepArgs<T> a) {
   if constexpr (!EXACT || !Def<ID>::linear) {
-    sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR>(a);  // no division in these forms
+    sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS>(a);  // no division in these forms
   } else if constexpr (Def<ID>::divisor != 0) {
-    if (a.has_div == 2) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 2, UNR>(a);
-    else if (a.has_div == 1) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 1, UNR>(a);
-    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR>(a);
+    if (a.has_div == 2) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 2, UNR, MIDS>(a);
+    else if (a.has_div == 1) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 1, UNR, MIDS>(a);
+    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS>(a);
   } else {
-    if (a.has_div == 0) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR>(a);
-    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, -1, UNR>(a);
+    if (a.has_div == 0) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS>(a);
+    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, -1, UNR, MIDS>(a);
   }
 }
 

@@ -32,7 +32,7 @@ def adj_btmax(name, dt):
     if d["rad"] == 1:
         return 4 if d["ntaps"] > 5 else 5
     if d["rad"] == 2:
-        return 2 if dt == np.float64 else 4
+        return 2 if dt == np.float64 else 3
     return 1
 
 

@@ -118,9 +118,15 @@ an5d_layout padded_layout(const Geom& g) {
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
 int device_cus() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
   int dev = 0, n = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
   if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  cache[dev] = n;
   return n;
 }
 
@@ -483,9 +489,16 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
     an5d_schedule(p->timesteps, v->kind == kDirect ? 1 : bt, steps.data(), n);
   }
   int c0 = c0_first;
+  // one plan per distinct step count (a schedule has at most three)
+  LaunchPlan plans[kMaxBT + 1];
+  bool planned[kMaxBT + 1] = {};
   for (int k : steps) {
-    LaunchPlan lp;
-    if (!plan_launch(*v, k, g, sl, lp, err, mode)) return fail(AN5D_E_UNSUPPORTED, err);
+    if (k < 1 || k > kMaxBT) return fail(AN5D_E_INVALID, "bad step count");
+    if (!planned[k]) {
+      if (!plan_launch(*v, k, g, sl, plans[k], err, mode)) return fail(AN5D_E_UNSUPPORTED, err);
+      planned[k] = true;
+    }
+    LaunchPlan lp = plans[k];
     char* base = static_cast<char*>(dA);
     lp.args.src = base + (size_t)(c0 % 2) * L.buf_stride * esz;
     lp.args.dst = base + (size_t)((c0 + 1) % 2) * L.buf_stride * esz;

@@ -330,7 +330,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   constexpr int XJ = MID ? R : 0;
   constexpr int XK = FAST ? R : 0;
   constexpr int H = BT * R;
-  constexpr bool STREAM = BT <= 2;  // non-temporal grid loads and stores
+  // Non-temporal grid loads where a launch is bandwidth-bound.  Stores stay
+  // temporal: a grid that fits the last-level cache is read back from it by
+  // the next launch.
+  constexpr bool STREAM = BT <= 2;
   static_assert(!FAST || CK >= R, "a lane must hold at least R cells");
   static_assert(DIM3 || CJ == 1, "2D sweeps hold one row per lane");
   static_assert(!MID || NW * CJ > 2 * H, "tile too thin for BT");
@@ -434,7 +437,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       if (!jst[jj]) continue;
       T* row = base + (DIM3 ? (long long)(jw + jj) * a.s1 : 0ll) + kl;
       if (kst_all) {
-        gstore_run<STREAM, T, CK>(row, pl.v[jj]);
+        gstore_run<false, T, CK>(row, pl.v[jj]);
       } else {
 #pragma unroll
         for (int kk = 0; kk < CK; kk++)

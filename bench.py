@@ -115,7 +115,7 @@ def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3):
     for abt in (2, 3, 4, 5):
         if abt > 1 and not d["linear"] and abt > 2:
             continue
-        for sl in (0, 128, 256):
+        for sl in (0, 64):
             try:
                 ws = pyan5d.AdjointWorkspace(name, n, T, np.float32, bt=abt, sl=sl)
             except pyan5d.An5dError:

@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -217,6 +218,31 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
   std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.model > b.model; });
   if ((int)cands.size() > max_measured) cands.resize(max_measured);
 
+  // Timing data must not decay into subnormals (the exact division then takes
+  // its slow path, and later candidates would be timed on different data):
+  // linear stencils are timed with their coefficients rescaled so that the
+  // signed taps sum to the divisor, which keeps the field's magnitude.  The
+  // division mode, and so the code path, is unchanged.
+  an5d_problem tp = *p;
+  std::vector<double> tcoef;
+  {
+    const StencilRT& s = stencil_rt(p->stencil);
+    if (s.linear) {
+      const double div = p->divisor < 0 ? (double)s.divisor : p->divisor;
+      double sum = 0.0;
+      tcoef.resize(s.ntaps);
+      for (int q = 0; q < s.ntaps; q++) {
+        tcoef[q] = p->coefs ? p->coefs[q] : (double)s.taps[q].coef;
+        sum += s.taps[q].sign > 0 ? tcoef[q] : -tcoef[q];
+      }
+      if (sum != 0.0 && std::isfinite(sum)) {
+        const double scale = (div != 0.0 ? div : 1.0) / sum;
+        for (double& c : tcoef) c *= scale;
+        tp.coefs = tcoef.data();
+      }
+    }
+  }
+
   // device buffers: the caller's, or a scratch grid of uniform [0,1) values
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   an5d_layout L = layout ? *layout : reference_layout(g);
@@ -241,7 +267,7 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
   int rc = AN5D_OK;
   for (Cand& c : cands) {
     an5d_config cfg{c.bt, c.variant, c.sl, mode};
-    an5d_problem q = *p;
+    an5d_problem q = tp;
     q.timesteps = c.bt;
     // warm up, then time repeated single launches (buffers alternate)
     rc = an5d_launch(&q, buf, &L, &cfg, c.bt, 0, nullptr);

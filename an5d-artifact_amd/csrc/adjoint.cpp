@@ -303,6 +303,7 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     lp.args.mid = mask ? ckpt(b0 + 1) : nullptr;
     lp.args.mid_stride = slot_elems;
     lp.args.mid_mask = mask;
+    lp.args.nt_store = 2 * ap.grid_bytes > kStreamBytes;
     e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
     if (e != hipSuccess) return hip_fail(e, "forward launch");
   }

@@ -507,6 +507,7 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
     lp.args.has_div = has_div;
     lp.args.s0 = L.s0;
     lp.args.s1 = L.s1;
+    lp.args.nt_store = (size_t)2 * L.buf_stride * esz > kStreamBytes;
     hipError_t e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     c0++;

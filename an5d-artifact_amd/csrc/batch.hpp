@@ -62,6 +62,7 @@ __device__ __forceinline__ void batch_tile(const BatchRec& r, int tile) {
   a.mid = nullptr;
   a.mid_stride = 0;
   a.mid_mask = 0;
+  a.nt_store = 0;
   constexpr int UNR = default_unroll<Def<ID>::rad>();
   if constexpr (Def<ID>::divisor != 0) {
     if (r.has_div == 2) sweep_body<ID, float, BT, kBatchCK, 1, kBatchNW, true, 2, UNR>(a, tile);

@@ -10,6 +10,9 @@ namespace an5d {
 
 constexpr int kMaxBT = 16;
 constexpr int kMaxMidBT = 8;  // deepest forward block with intermediate-stage outputs
+// grids (both buffers) above this size stream their stores past the 256 MB
+// last-level cache
+constexpr size_t kStreamBytes = 512ull << 20;
 
 enum KernelKind { kSweep = 0, kDirect = 1 };
 enum DType { kF32 = 0, kF64 = 1 };
@@ -28,6 +31,7 @@ struct LaunchArgs {
   void* mid;             // intermediate-stage outputs (see SweepArgs::mid_mask)
   long long mid_stride;  // elements between consecutive intermediate outputs
   unsigned mid_mask;     // bit S: store stage S; 0: none
+  int nt_store;          // 1: non-temporal stores for shallow blocks
 };
 
 using LaunchFn = hipError_t (*)(const LaunchArgs&, dim3 grid, dim3 block, size_t lds,

@@ -30,6 +30,7 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
   s.mid = static_cast<U*>(a.mid);
   s.mid_stride = a.mid_stride;
   s.mid_mask = a.mid_mask;
+  s.nt_store = a.nt_store;
   return s;
 }
 
@@ -73,10 +74,20 @@ hipError_t launch_direct(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds,
   return hipGetLastError();
 }
 
+// The shape's btmax lowered until the exchange buffers fit gfx950's 160 KB of LDS.
+template <int W>
+constexpr Shape fitted_shape() {
+  Shape sh = sweep_shape<SID, T>(W);
+  while (sh.btmax > 0 && sweep_lds_bytes<SID, T, sweep_shape<SID, T>(W).ck, sweep_shape<SID, T>(W).cj,
+                                         sweep_shape<SID, T>(W).nw>(sh.btmax) > 160 * 1024)
+    sh.btmax--;
+  return sh;
+}
+
 template <int W>
 void add_sweep(std::vector<Variant>& out) {
-  constexpr Shape sh = sweep_shape<SID, T>(W);
-  if constexpr (sh.btmax > 0 && sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>(sh.btmax) <= 160 * 1024) {
+  constexpr Shape sh = fitted_shape<W>();
+  if constexpr (sh.btmax > 0) {
     Variant v{};
     v.stencil = SID;
     v.dtype = AN5D_F64 ? kF64 : kF32;

@@ -51,6 +51,7 @@ struct SweepArgs {
   T* __restrict__ mid;
   long long mid_stride;
   unsigned mid_mask;
+  int nt_store;  // 1: non-temporal stores (grid larger than the last-level cache)
 };
 
 // ---- cross-lane moves ------------------------------------------------------
@@ -298,13 +299,24 @@ constexpr bool star_exchange() {
          !needs_corners<D>();
 }
 
-// LDS bytes of the middle-dim exchange: stars one buffer with a slot per
-// stage, the others two buffers of one slot per extended plane.
+// Radius-1 stars with bt >= 3 double-buffer that exchange: the edge rows a
+// stage produces in one iteration go to the other buffer right away, and one
+// barrier per iteration separates them from their readers.  (Shallower
+// blocks are bandwidth-bound and keep the single buffer, which leaves room
+// for more resident workgroups.)
+template <class D>
+constexpr bool star_exchange_db(int bt) {
+  return star_exchange<D>() && D::rad == 1 && bt >= 3;
+}
+
+// LDS bytes of the middle-dim exchange: stars one buffer (two for radius 1)
+// with a slot per stage, the others two buffers of one slot per extended plane.
 template <int ID, typename T, int CK, int CJ, int NW>
 constexpr size_t sweep_lds_bytes(int bt) {
   using D = Def<ID>;
   if constexpr (star_exchange<D>())
-    return (size_t)bt * NW * edge_rows<CJ, D::rad>() * 64 * CK * sizeof(T);
+    return (star_exchange_db<D>(bt) ? 2 : 1) * (size_t)bt * NW * edge_rows<CJ, D::rad>() * 64 * CK *
+           sizeof(T);
   else if constexpr (D::dim == 3 && needs_mid<D>())
     return 2ull * num_ext_planes<D>() * NW * edge_rows<CJ, D::rad>() * 64 * CK * sizeof(T);
   else
@@ -330,9 +342,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   constexpr int XJ = MID ? R : 0;
   constexpr int XK = FAST ? R : 0;
   constexpr int H = BT * R;
-  // Non-temporal grid loads where a launch is bandwidth-bound.  Stores stay
-  // temporal: a grid that fits the last-level cache is read back from it by
-  // the next launch.
+  // Non-temporal grid loads where a launch is bandwidth-bound; non-temporal
+  // stores too when the grid does not fit the last-level cache (otherwise the
+  // next launch reads the output back from it).
   constexpr bool STREAM = BT <= 2;
   static_assert(!FAST || CK >= R, "a lane must hold at least R cells");
   static_assert(DIM3 || CJ == 1, "2D sweeps hold one row per lane");
@@ -437,7 +449,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       if (!jst[jj]) continue;
       T* row = base + (DIM3 ? (long long)(jw + jj) * a.s1 : 0ll) + kl;
       if (kst_all) {
-        gstore_run<false, T, CK>(row, pl.v[jj]);
+        if (STREAM && a.nt_store) gstore_run<true, T, CK>(row, pl.v[jj]);
+        else gstore_run<false, T, CK>(row, pl.v[jj]);
       } else {
 #pragma unroll
         for (int kk = 0; kk < CK; kk++)
@@ -583,6 +596,23 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   Plane<T, CJ, CK> pre[PF];
 #pragma unroll
   for (int f = 0; f < PF; f++) load_plane(pre[f], i0 - H + f);
+  // radius-1 stars: edge rows of plane `pl` as stage slot `st`'s centre plane
+  // of the next iteration, into exchange buffer `b`
+  constexpr bool STARX_DB = star_exchange_db<D>(BT);
+  constexpr size_t XBUF = (size_t)BT * NW * ERS * 64 * CK;  // elements per buffer
+  auto publish = [&](const Plane<T, CJ, CK>& pl, int st, int b) {
+    T* lds = reinterpret_cast<T*>(an5d_lds) + (size_t)b * XBUF;
+#pragma unroll
+    for (int jj = 0; jj < CJ; jj++) {
+      const int e = edge_index<CJ, XJ>(jj);
+      if (e >= 0) store_run<T, CK>(lds + ((((size_t)st * NW + w) * ERS + e) * 64 + lane) * CK, pl.v[jj]);
+    }
+  };
+  if constexpr (STARX_DB) {
+    // iteration 0 reads buffer 0: the (warm-up) centre planes of the rings
+#pragma unroll
+    for (int st = 0; st < BT; st++) publish(ring[st][R + 1], st, 0);
+  }
   const int nit_pad = (nit + UNR - 1) / UNR * UNR;
   for (int n0 = 0; n0 < nit_pad; n0 += UNR) {
 #pragma unroll
@@ -590,7 +620,26 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       const int n = n0 + u;
       const int p_in = i0 - H + n;
       const int slot = PF_ROT ? u % PF : 0;
-      if constexpr (STARX) {
+      if constexpr (STARX_DB) {
+        // the edge rows of iteration n were written during iteration n-1
+        // into buffer n&1; one barrier, then read the neighbours' rows
+        __syncthreads();
+        const T* lds = reinterpret_cast<const T*>(an5d_lds) + (size_t)(n & 1) * XBUF;
+        static_for<1, BT + 1>([&](auto sc) {
+          constexpr int S = decltype(sc)::value;
+          static_for<0, 2 * XJ>([&](auto hc) {
+            constexpr int HI = decltype(hc)::value;
+            constexpr int JR = HI < XJ ? HI - XJ : CJ + (HI - XJ);
+            constexpr int WOFF = JR >= 0 ? JR / CJ : -((-JR + CJ - 1) / CJ);
+            constexpr int RR = JR - WOFF * CJ;
+            constexpr int E = edge_index<CJ, XJ>(RR);
+            static_assert(E >= 0, "halo row must be published");
+            int ws = w + WOFF;
+            ws = ws < 0 ? 0 : (ws >= NW ? NW - 1 : ws);
+            load_run<T, CK>(xh[S - 1][HI], lds + ((((size_t)(S - 1) * NW + ws) * ERS + E) * 64 + lane) * CK);
+          });
+        });
+      } else if constexpr (STARX) {
         // Every stage's centre plane this iteration is slot R+1 of its input
         // ring before the rotation: publish their edge rows, one barrier,
         // read the neighbours' rows, a second barrier before the next write.
@@ -630,10 +679,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         for (int f = 0; f + 1 < PF; f++) pre[f] = pre[f + 1];
       }
       load_plane(pre[PF_ROT ? slot : PF - 1], p_in + PF);
+      if constexpr (STARX_DB) publish(ring[0][P - 1], 0, (n + 1) & 1);
       static_for<1, BT + 1>([&](auto sc) {
         constexpr int S = decltype(sc)::value;
         Plane<T, CJ, CK> out;
         compute(ring[S - 1], out, p_in - S * R, sc);
+        if constexpr (STARX_DB && S < BT) publish(out, S, (n + 1) & 1);
         if constexpr (S < BT) {
           // owned planes of stage S: n in [H + S*R, H + S*R + sle)
           if constexpr (MIDS) {

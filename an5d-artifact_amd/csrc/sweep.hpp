@@ -18,9 +18,11 @@
 //     spreads R cells per stage: a wave therefore produces 64*CK - 2*BT*R
 //     valid cells (overlapped tiling).
 //   * 3D only: the middle dim is split over the NW waves of a workgroup, CJ
-//     rows per lane.  Rows owned by another wave come through LDS (one
-//     double-buffered exchange and one barrier per stage that needs them).
-//     In 2D the NW waves of a workgroup are independent tiles.
+//     rows per lane.  Rows owned by another wave come through LDS: for stars
+//     one exchange per input plane covers every stage (radius 1 with bt >= 3:
+//     double-buffered, one barrier), boxes exchange per stage.  In 2D the NW
+//     waves of a workgroup are independent tiles.
+//   * input planes are queued PF iterations ahead in registers.
 //
 // Exactness: each cell of each stage is produced by the same expression, in
 // the same term order, from the same operands as the C loop (no contraction:

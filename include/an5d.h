@@ -151,15 +151,22 @@ int an5d_init_grid(int dtype, void* A, long long cells, long long seed_base, int
  * dL/du_0 on the interior (the halo is a fixed boundary condition: its
  * entries are 0), dL/dcoefs (one per tap in source order; gradient2d:
  * {d/d eps, d/d scale}) and L.  Single grids (one buffer, D0 x D1 cells,
- * reference layout).  The forward pass keeps a checkpoint at every launch
- * boundary (bt steps); each reverse launch recomputes the states of its
- * block from the checkpoint and runs the transposed sweep for bt steps. */
+ * reference layout).  cfg->bt is the adjoint block (0: deepest compiled),
+ * cfg->variant / cfg->sl the forward kernel and the adjoint segment length.
+ * The forward pass runs at its own tuned block depth (a multiple of bt) and
+ * stores a checkpoint every bt steps; each reverse launch recomputes the
+ * states of its block from the checkpoint and runs the transposed sweep for
+ * bt steps.  The gradient and loss sums are reduced in a fixed order, so
+ * repeated calls give identical results. */
 int an5d_adjoint_workspace(const an5d_problem* prob, const an5d_config* cfg, size_t* bytes);
 /* Device form: enqueues on `stream`; *ms_fwd (optional) gets the time of the
  * checkpointed forward pass (synchronises on an event when non-NULL).
  * gcoef and loss are device doubles.  The workspace (size from
- * an5d_adjoint_workspace) must be zero-filled once before its first use;
- * the library keeps the parts it relies on zero afterwards. */
+ * an5d_adjoint_workspace for the same problem and cfg) holds the checkpoints,
+ * two adjoint buffers and per-wave partial sums; its grids carry zero padding
+ * around them, so it must be zero-filled once before its first use (the
+ * library never writes the padding).  Calls that share a workspace must not
+ * overlap. */
 int an5d_adjoint_device(const an5d_problem* prob, const void* u0, const void* target, void* grad,
                         double* gcoef, double* loss, void* workspace, size_t workspace_bytes,
                         const an5d_config* cfg, void* stream, double* ms_fwd);

@@ -71,3 +71,12 @@ def test_resolve_config_follows_flags_then_table_then_defaults(tmp_path):
     pyan5d.tune_clear()
     with pytest.raises(pyan5d.An5dError):
         pyan5d.resolve_config("j2d5pt", [1000, 1000], variant=99)
+
+
+def test_tune_workload_sets_cover_every_stencil():
+    from pyan5d import tune
+    names = {w[0] for w in tune.workloads("all")}
+    assert names == set(tune.STENCILS_2D + tune.STENCILS_3D)
+    assert len(names) == pyan5d.lib().an5d_stencil_count()
+    bench = tune.workloads("bench")
+    assert ("star3d1r", np.float32, [512, 512, 512], (pyan5d.EXACT, pyan5d.FAST)) in bench

@@ -51,10 +51,9 @@ struct AdjPlan {
   const Variant* fwd = nullptr;
   const AdjVariant* adj = nullptr;
   int bt = 0, sl = 0;    // adjoint block size and segment length
-  int btf = 0, fsl = 0;  // forward launches: steps per launch (a multiple of bt) and sl
+  int btf = 0, fsl = 0;  // forward launches: steps per launch and sl
   std::vector<int> fsteps;  // forward launch schedule
-  std::vector<int> blocks;  // adjoint blocks in time order (<= bt steps each)
-  std::vector<int> fblock;  // first block of each forward launch
+  std::vector<int> blocks;  // adjoint blocks in time order (bt steps, the last may be shorter)
   Geom g;
   long long pitch = 0;  // elements per padded row
   size_t grid_bytes = 0, ws_bytes = 0;
@@ -69,9 +68,10 @@ struct AdjPlan {
 static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block,
                            bool sizing = false);
 
-// The forward pass runs launches of btf = m*bt steps that also store the
-// states at every bt-th stage, so the checkpoints sit bt steps apart while the
-// forward sweep keeps its own (deeper) temporal block.
+// The forward pass runs at its own temporal block btf (the multiple of bt
+// nearest the tuned forward depth) and stores the state at every multiple of
+// bt: a launch's output is the checkpoint where it ends, and the intermediate
+// stages that reach a block boundary inside it store theirs.
 static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& ap) {
   std::string err;
   if (!make_geom(p, ap.g, err)) return fail(AN5D_E_INVALID, err);
@@ -93,21 +93,21 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   }
   if (ap.fwd->kind == kDirect) bt = 1;
   ap.bt = bt;
+  // every launch ends on a block boundary (bt <= btmax, so m = 1 always fits)
   int m = (bt_pref + bt / 2) / bt;
   if (m < 1) m = 1;
   while (m > 1 && (m * bt > ap.fwd->btmax || m * bt > kMaxMidBT)) m--;
-  ap.btf = m * bt;
-  ap.fsl = ap.btf == bt_pref ? sl_pref : 0;
-  // plain split (the checkpoints are separate buffers, so the ping-pong
-  // parity rule of the forward schedule does not apply)
-  ap.fsteps.assign(p->timesteps / ap.btf, ap.btf);
-  if (p->timesteps % ap.btf) ap.fsteps.push_back(p->timesteps % ap.btf);
+  int btf = m * bt;
+  if (ap.fwd->kind == kDirect) btf = 1;
+  ap.btf = btf;
+  ap.fsl = btf == bt_pref ? sl_pref : 0;
+  // plain split (the states go to separate buffers, so the ping-pong parity
+  // rule of the forward schedule does not apply)
+  const int T = p->timesteps;
+  ap.fsteps.assign(T / btf, btf);
+  if (T % btf) ap.fsteps.push_back(T % btf);
   ap.blocks.clear();
-  ap.fblock.clear();
-  for (int kf : ap.fsteps) {
-    ap.fblock.push_back((int)ap.blocks.size());
-    for (int t = 0; t < kf; t += bt) ap.blocks.push_back(kf - t < bt ? kf - t : bt);
-  }
+  for (int t = 0; t < T; t += bt) ap.blocks.push_back(T - t < bt ? T - t : bt);
   ap.sl = cfg ? cfg->sl : 0;
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   ap.pitch = (ap.g.D2 + 2 * kPad + 63) / 64 * 64;
@@ -278,34 +278,41 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     e = ap.adj->halo(ckpt(0), ckpt(i), (int)g.D0, (int)g.D2, P, R, st);
     if (e != hipSuccess) return hip_fail(e, "halo copy");
   }
-  // forward: launch j advances blocks fblock[j] .. fblock[j]+nb-1 and stores
-  // the state at every block boundary (the last one as its output)
+  // forward: launch j advances [t0, t0+kf); the state at every multiple of
+  // bt inside it goes to that block's checkpoint
   const long long slot_elems = (long long)(ap.grid_bytes / esz);
+  const int bt = ap.bt, T = p->timesteps;
+  int t0 = 0;
   for (size_t j = 0; j < ap.fsteps.size(); j++) {
-    const int kf = ap.fsteps[j];
-    const int b0 = ap.fblock[j];
-    const int nb = (kf + ap.bt - 1) / ap.bt;
+    const int kf = ap.fsteps[j], t1 = t0 + kf;
     LaunchPlan lp;
     if (!plan_launch(*ap.fwd, kf, g, ap.fsl, lp, err)) return fail(AN5D_E_UNSUPPORTED, err);
-    lp.args.src = ckpt(b0);
-    lp.args.dst = ckpt(b0 + nb);
+    if (t0 % bt != 0 || (t1 != T && t1 % bt != 0))
+      return fail(AN5D_E_UNSUPPORTED, "forward launch does not end on a block boundary");
+    lp.args.src = ckpt(t0 / bt);
+    lp.args.dst = t1 == T ? ckpt(L) : ckpt(t1 / bt);
     lp.args.coef = dcoef;
     lp.args.gscale = gscale;
     lp.args.has_div = has_div;
     lp.args.s0 = P;
     lp.args.s1 = P;
     unsigned mask = 0;
-    for (int t = 1; t < nb; t++) mask |= 1u << (t * ap.bt);
+    int bfirst = -1;
+    for (int b = t0 / bt + 1; b * bt < t1; b++) {
+      mask |= 1u << (b * bt - t0);
+      if (bfirst < 0) bfirst = b;
+    }
     if (mask) {
       lp.fn = ap.fwd->fn_mid[kf];
       if (!lp.fn) return fail(AN5D_E_UNSUPPORTED, "no forward kernel with intermediate outputs");
     }
-    lp.args.mid = mask ? ckpt(b0 + 1) : nullptr;
+    lp.args.mid = mask ? ckpt(bfirst) : nullptr;
     lp.args.mid_stride = slot_elems;
     lp.args.mid_mask = mask;
     lp.args.nt_store = 2 * ap.grid_bytes > kStreamBytes;
     e = lp.fn(lp.args, lp.grid, lp.block, lp.lds, st);
     if (e != hipSuccess) return hip_fail(e, "forward launch");
+    t0 = t1;
   }
   if (ms_fwd) (void)hipEventRecord(ev1, st);
   // seed: lambda_T = u_T - target on the interior (0 on the halo), loss slots

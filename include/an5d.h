@@ -153,8 +153,8 @@ int an5d_init_grid(int dtype, void* A, long long cells, long long seed_base, int
  * {d/d eps, d/d scale}) and L.  Single grids (one buffer, D0 x D1 cells,
  * reference layout).  cfg->bt is the adjoint block (0: deepest compiled),
  * cfg->variant / cfg->sl the forward kernel and the adjoint segment length.
- * The forward pass runs at its own tuned block depth (a multiple of bt) and
- * stores a checkpoint every bt steps; each reverse launch recomputes the
+ * The forward pass runs at its own tuned block depth and stores a checkpoint
+ * every bt steps; each reverse launch recomputes the
  * states of its block from the checkpoint and runs the transposed sweep for
  * bt steps.  The gradient and loss sums are reduced in a fixed order, so
  * repeated calls give identical results. */

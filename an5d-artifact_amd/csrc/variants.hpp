@@ -62,7 +62,11 @@ constexpr Shape sweep_shape(int which) {
     }
     if constexpr (R >= 3) {
       if (is_box<D>()) return Shape{0, 0, 0, 0};
-      if (which == 0) return Shape{4, 2, 16, R == 3 ? 2 : 1};
+      // FP32: 8 waves of 3-4 rows keep the (2R+1)-plane rings in 256 VGPRs
+      // (16-wave workgroups cap a wave at 128 and spill); FP64 keeps the
+      // 16-wave shape
+      if (!f64 && which == 0) return R == 3 ? Shape{4, 4, 8, 1} : Shape{4, 3, 8, 1};
+      if (which == (f64 ? 0 : 1)) return Shape{4, 2, 16, R == 3 ? 2 : 1};
       return Shape{0, 0, 0, 0};
     }
     return Shape{0, 0, 0, 0};

@@ -593,7 +593,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   // count is padded to a multiple of UNR so the unrolled body has no exit
   // inside it: the ring rotation then costs no register moves.  The padding
   // iterations only produce planes past the segment, never stored.
-  constexpr int PF = prefetch_depth<UNR, CJ * CK * (int)(sizeof(T) / 4)>();
+  // 3D: a plane is CJ rows and the LDS exchange paces the loop; a shallow
+  // queue keeps the registers for the rings
+  constexpr int PF = prefetch_depth<UNR, CJ * CK * (int)(sizeof(T) / 4) * (DIM3 ? 3 : 1)>();
   constexpr bool PF_ROT = UNR % PF == 0;  // queue slots rotate by renaming
   Plane<T, CJ, CK> pre[PF];
 #pragma unroll

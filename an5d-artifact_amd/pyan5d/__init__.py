@@ -365,10 +365,12 @@ class Batch:
             pass
 
 
-def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None):
+def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None, timesteps=1):
     """Rank configurations with the model, time the best on the GPU, record
-    the winner.  Returns (config dict, GStencil/s)."""
-    p, keep = make_problem(name, n, 1, dtype)
+    the winner.  `timesteps`: the length of the run being tuned for (the
+    candidates are timed on the field halfway through it).  Returns (config
+    dict, GStencil/s)."""
+    p, keep = make_problem(name, n, timesteps, dtype)
     best = Config()
     gs = ctypes.c_double()
     ptr, lay = None, None

@@ -2,9 +2,10 @@
 
     python -m pyan5d.tune [--set bench|all] [--out FILE] [--max-measured K]
 
-For every (stencil, dtype, size, mode) of the set, an5d_autotune ranks all
-(kernel shape, bt, sl) candidates with the analytic model, times the best K on
-the GPU and keeps the fastest; the table is then written to FILE (default:
+For every (stencil, dtype, size, mode, run length) of the set, an5d_autotune
+ranks all (kernel shape, bt, sl) candidates with the analytic model, times the
+best K on the GPU (on the field halfway through the run) and keeps the
+fastest; the table is then written to FILE (default:
 the file the library loads, an5d-artifact_amd/tuned.tsv).  This replaces the
 reference's Excel model + top-5 compile/run scripts
 (compiled/nvcc_compile_top5.sh, compiled/run.sh), which pick one
@@ -26,33 +27,33 @@ STENCILS_2D = ["star2d1r", "star2d2r", "star2d3r", "star2d4r", "box2d1r", "box2d
 STENCILS_3D = ["star3d1r", "star3d2r", "star3d3r", "star3d4r", "box3d1r", "box3d2r", "box3d3r",
                "box3d4r", "j3d27pt"]
 
-# (stencil, dtype, interior size, modes)
+# (stencil, dtype, interior size, modes, timesteps of the run)
 BENCH = [
-    ("star3d1r", np.float32, [512, 512, 512], (EXACT, FAST)),
-    ("star3d1r", np.float64, [512, 512, 512], (EXACT, FAST)),
-    ("j2d5pt", np.float32, [16384, 16384], (EXACT, FAST)),
-    ("j2d5pt", np.float32, [4096, 4096], (EXACT, FAST)),
-    ("box2d4r", np.float32, [16384, 16384], (EXACT,)),
-    ("box2d4r", np.float64, [16384, 16384], (EXACT,)),
-    ("star3d4r", np.float32, [512, 512, 512], (EXACT,)),
-    ("star3d4r", np.float64, [512, 512, 512], (EXACT,)),
-    ("j3d27pt", np.float32, [512, 512, 512], (EXACT,)),
-    ("j3d27pt", np.float64, [512, 512, 512], (EXACT,)),
-    ("gradient2d", np.float32, [8192, 8192], (EXACT,)),
-    ("j2d9pt", np.float32, [8192, 8192], (EXACT,)),
-    ("j2d9pt-gol", np.float32, [8192, 8192], (EXACT,)),
+    ("star3d1r", np.float32, [512, 512, 512], (EXACT, FAST), 1000),
+    ("star3d1r", np.float64, [512, 512, 512], (EXACT, FAST), 1000),
+    ("j2d5pt", np.float32, [16384, 16384], (EXACT, FAST), 1000),
+    ("j2d5pt", np.float32, [4096, 4096], (EXACT, FAST), 100),
+    ("box2d4r", np.float32, [16384, 16384], (EXACT,), 1000),
+    ("box2d4r", np.float64, [16384, 16384], (EXACT,), 1000),
+    ("star3d4r", np.float32, [512, 512, 512], (EXACT,), 1000),
+    ("star3d4r", np.float64, [512, 512, 512], (EXACT,), 1000),
+    ("j3d27pt", np.float32, [512, 512, 512], (EXACT,), 1000),
+    ("j3d27pt", np.float64, [512, 512, 512], (EXACT,), 1000),
+    ("gradient2d", np.float32, [8192, 8192], (EXACT,), 256),
+    ("j2d9pt", np.float32, [8192, 8192], (EXACT,), 256),
+    ("j2d9pt-gol", np.float32, [8192, 8192], (EXACT,), 256),
 ]
 
 
 def workloads(which):
     out = list(BENCH)
     if which == "all":
-        seen = {(s, np.dtype(d).name, tuple(n), m) for s, d, n, ms in BENCH for m in ms}
+        seen = {(s, np.dtype(d).name, tuple(n), m) for s, d, n, ms, _ in BENCH for m in ms}
         for s in STENCILS_2D + STENCILS_3D:
             n = [16384, 16384] if s in STENCILS_2D else [512, 512, 512]
             for dt in (np.float32, np.float64):
                 if (s, np.dtype(dt).name, tuple(n), EXACT) not in seen:
-                    out.append((s, dt, n, (EXACT,)))
+                    out.append((s, dt, n, (EXACT,), 1000))
     return out
 
 
@@ -67,11 +68,11 @@ def main(argv=None):
         tune_clear()
     elif args.out and os.path.exists(args.out):
         tune_load(args.out)
-    for name, dt, n, modes in workloads(args.set):
+    for name, dt, n, modes, steps in workloads(args.set):
         info(name)  # validates the name
         for mode in modes:
             t0 = time.perf_counter()
-            cfg, gs = autotune(name, n, dt, mode=mode, max_measured=args.max_measured)
+            cfg, gs = autotune(name, n, dt, mode=mode, max_measured=args.max_measured, timesteps=steps)
             print(f"[tune] {name} {np.dtype(dt).name} {'x'.join(map(str, n))} "
                   f"{'fast' if mode == FAST else 'exact'}: variant {cfg['variant']} bt {cfg['bt']} "
                   f"sl {cfg['sl']} -> {gs:.1f} GStencil/s ({time.perf_counter() - t0:.1f} s)", flush=True)

@@ -79,4 +79,4 @@ def test_tune_workload_sets_cover_every_stencil():
     assert names == set(tune.STENCILS_2D + tune.STENCILS_3D)
     assert len(names) == pyan5d.lib().an5d_stencil_count()
     bench = tune.workloads("bench")
-    assert ("star3d1r", np.float32, [512, 512, 512], (pyan5d.EXACT, pyan5d.FAST)) in bench
+    assert ("star3d1r", np.float32, [512, 512, 512], (pyan5d.EXACT, pyan5d.FAST), 1000) in bench

@@ -115,6 +115,7 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   add_sweep<0>(out);
   add_sweep<1>(out);
   add_sweep<2>(out);
+  add_sweep<3>(out);
   Variant d{};
   d.stencil = SID;
   d.dtype = AN5D_F64 ? kF64 : kF32;

@@ -110,10 +110,12 @@ template <typename T, int N>
 struct NtVec;
 template <> struct NtVec<float, 1> { typedef float type __attribute__((aligned(4))); };
 template <> struct NtVec<float, 2> { typedef float type __attribute__((ext_vector_type(2), aligned(4))); };
+template <> struct NtVec<float, 3> { typedef float type __attribute__((ext_vector_type(3), aligned(4))); };
 template <> struct NtVec<float, 4> { typedef float type __attribute__((ext_vector_type(4), aligned(4))); };
 template <> struct NtVec<float, 8> { typedef float type __attribute__((ext_vector_type(8), aligned(4))); };
 template <> struct NtVec<double, 1> { typedef double type __attribute__((aligned(4))); };
 template <> struct NtVec<double, 2> { typedef double type __attribute__((ext_vector_type(2), aligned(4))); };
+template <> struct NtVec<double, 3> { typedef double type __attribute__((ext_vector_type(3), aligned(4))); };
 template <> struct NtVec<double, 4> { typedef double type __attribute__((ext_vector_type(4), aligned(4))); };
 template <> struct NtVec<double, 8> { typedef double type __attribute__((ext_vector_type(8), aligned(4))); };
 

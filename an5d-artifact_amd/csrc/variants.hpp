@@ -17,7 +17,7 @@ struct Shape {
   int ck, cj, nw, btmax;
 };
 
-constexpr int kMaxShapes = 3;
+constexpr int kMaxShapes = 4;
 
 template <class D>
 constexpr bool is_box() {
@@ -53,6 +53,9 @@ constexpr Shape sweep_shape(int which) {
       if (which == 0) return Shape{2, 4, 8, 6};
       if (which == 1) return Shape{4, 4, 4, 4};
       if (which == 2) return Shape{2, 2, 16, 6};
+      // 192-cell rows (3 per lane): 512 = 3 tiles of 186 at bt 3 instead of
+      // 5 of 122 (FP32, within 128 VGPRs)
+      if (which == 3 && !f64) return Shape{3, 2, 16, 3};
       return Shape{0, 0, 0, 0};
     }
     if constexpr (R == 2) {

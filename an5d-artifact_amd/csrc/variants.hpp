@@ -48,6 +48,7 @@ constexpr Shape sweep_shape(int which) {
       if (is_box<D>()) {
         if (which == 0) return Shape{2, 4, 8, 4};
         if (which == 1) return Shape{4, 2, 8, 3};
+        if (which == 2 && !f64) return Shape{3, 4, 8, 3};  // 512 = three 192-cell tiles
         return Shape{0, 0, 0, 0};
       }
       if (which == 0) return Shape{2, 4, 8, 6};
@@ -61,6 +62,7 @@ constexpr Shape sweep_shape(int which) {
     if constexpr (R == 2) {
       if (is_box<D>()) return Shape{0, 0, 0, 0};  // direct kernel only
       if (which == 0) return Shape{2, 4, 8, 3};
+      if (which == 1 && !f64) return Shape{3, 4, 8, 2};
       return Shape{0, 0, 0, 0};
     }
     if constexpr (R >= 3) {

@@ -240,8 +240,10 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
   // favourite): with the source coefficients many fields decay, and a field
   // that has settled into zeros with thin subnormal bands at the boundary
   // (where the exact division takes its IEEE path) favours other tilings
-  // than the random initial field.  Once settled the field is stationary, so
-  // every candidate sees the same data.
+  // than the random initial field.  Every candidate starts from a copy of
+  // that field, so all of them see the same data.
+  void* snap = nullptr;
+  const size_t grid_bytes = (size_t)2 * L.buf_stride * esz;
   {
     int warm = p->timesteps / 2;
     if (warm > 512) warm = 512;
@@ -256,6 +258,14 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
         return rc0;
       }
     }
+    // without room for the copy the candidates run on in sequence
+    if (hipMalloc(&snap, grid_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      snap = nullptr;
+    } else if (hipMemcpy(snap, buf, grid_bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
+      (void)hipFree(snap);
+      snap = nullptr;
+    }
   }
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
@@ -266,6 +276,10 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
     an5d_config cfg{c.bt, c.variant, c.sl, mode};
     an5d_problem q = *p;
     q.timesteps = c.bt;
+    if (snap) {
+      e = hipMemcpyAsync(buf, snap, grid_bytes, hipMemcpyDeviceToDevice, nullptr);
+      if (e != hipSuccess) { rc = hip_fail(e, "autotune restore"); break; }
+    }
     // warm up, then time repeated single launches (buffers alternate)
     rc = an5d_launch(&q, buf, &L, &cfg, c.bt, 0, nullptr);
     if (rc) break;
@@ -289,6 +303,7 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (snap) (void)hipFree(snap);
   if (owned) (void)hipFree(owned);
   if (rc) return rc;
   const Cand* win = &cands[0];

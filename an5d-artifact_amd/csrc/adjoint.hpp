@@ -77,16 +77,6 @@ __device__ __forceinline__ void extend_row(const T (&src)[CK], T (&x)[CK + 2 * X
   }
 }
 
-// Stage input getter over row-extended planes (2D): plane A0 relative to the
-// centre, fastest-dim offset A2.
-template <class D, typename T, int NP, int CK, int XK, int KK>
-struct RowGet {
-  const T (&x)[NP][CK + 2 * XK];
-  int centre;
-  template <int A0, int A1, int A2>
-  __device__ __forceinline__ T at() const { return x[D::rad + A0][KK + A2 + XK]; }
-};
-
 template <typename T>
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -120,7 +110,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   if (gw >= a.ntk * a.nseg) return;
   const int tk = gw % a.ntk, seg = gw / a.ntk;
   const int kv0 = R + tk * a.SK;
-  const int kv1 = min(kv0 + a.SK, a.D2 - R);
   const int kl = kv0 - HA + lane * CK;
   const int i0 = R + seg * a.sl;
   const int sle = min(a.sl, a.D0 - R - i0);
@@ -139,7 +128,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   // the lane's cells belong to this tile (lane-aligned: HA and SK are whole
   // lanes); in the last tile the owned lanes past the interior hold lambda = 0
   const bool lown = kl >= kv0 && kl < kv0 + a.SK;
-  (void)kv1;
 
   T c[NT + 2];
 #pragma unroll
@@ -244,7 +232,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
         T out[CK];
         static_for<0, CK>([&](auto kc) {
           constexpr int KK = decltype(kc)::value;
-          RowGet<D, T, P, CK, XK, KK> get{xin, R};
           T val;
           if constexpr (D::linear) {
             // folded coefficients s_q c_q / div, FMA chain
@@ -262,7 +249,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
             const T dn = cc - xin[R - 1][KK + XK], ds = cc - xin[R + 1][KK + XK];
             const T de = cc - xin[R][KK + XK + 1], dw = cc - xin[R][KK + XK - 1];
             val = cc + (T)gscale * rsqrt_t(c[0] + dn * dn + ds * ds + de * de + dw * dw);
-            (void)get;
           }
           out[KK] = val;
         });

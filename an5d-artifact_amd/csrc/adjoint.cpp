@@ -93,8 +93,10 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   }
   if (ap.fwd->kind == kDirect) bt = 1;
   ap.bt = bt;
-  // every launch ends on a block boundary (bt <= btmax, so m = 1 always fits)
-  int m = (bt_pref + bt / 2) / bt;
+  // every launch ends on a block boundary (bt <= btmax, so m = 1 always fits);
+  // a tie goes to the smaller m: a launch that stores no intermediate state is
+  // cheaper per step than one that does at the same distance from bt_pref
+  int m = (bt_pref + (bt - 1) / 2) / bt;
   if (m < 1) m = 1;
   while (m > 1 && (m * bt > ap.fwd->btmax || m * bt > kMaxMidBT)) m--;
   int btf = m * bt;

@@ -81,7 +81,7 @@ def cpu_baseline(wl, budget_s=12.0):
             "kind": "port", "sample": f"oracle sequential loop, {'x'.join(map(str, n))}, {steps} steps"}
 
 
-def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3):
+def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3, abts=(2, 3, 4, 5), sls=(0, 64)):
     """(checkpointed forward + adjoint) / plain forward, 2D, FP32."""
     n = list(n)
     d = pyan5d.info(name)
@@ -112,10 +112,10 @@ def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3):
     gc = torch.zeros(d["ntaps"] if d["linear"] else 2, dtype=torch.float64, device="cuda")
     loss = torch.zeros(1, dtype=torch.float64, device="cuda")
     best_fa = None
-    for abt in (2, 3, 4, 5):
+    for abt in abts:
         if abt > 1 and not d["linear"] and abt > 2:
             continue
-        for sl in (0, 64):
+        for sl in sls:
             try:
                 ws = pyan5d.AdjointWorkspace(name, n, T, np.float32, bt=abt, sl=sl)
             except pyan5d.An5dError:
@@ -216,6 +216,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (scale reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--fwd-adj-only", default=None, metavar="STENCIL",
+                    help="run only the forward+adjoint leg of one 2D stencil and print its record "
+                         "(profiling runs); --adj-bt limits it to one adjoint bt with the default sl")
+    ap.add_argument("--adj-bt", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -227,6 +231,13 @@ def main():
     import torch
     import pyan5d
     from pyan5d import presets
+
+    if args.fwd_adj_only:
+        abts = (args.adj_bt,) if args.adj_bt > 0 else (2, 3, 4, 5)
+        rec = bench_fwd_adj(torch, pyan5d, presets, args.fwd_adj_only, abts=abts,
+                            sls=(0,) if args.adj_bt > 0 else (0, 64))
+        print(json.dumps({"fwd_adj": rec}), flush=True)
+        return
 
     wl = WORKLOADS[args.workload]
     name, dt, n, T = wl["stencil"], wl["dtype"], wl["n"], wl["T"]

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -62,6 +63,15 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C an5d-artifact_amd` "
                                "(there is no fallback path)")
+        # torch's ROCm wheel carries its own libamdhip64.so.  Loaded first, it
+        # also serves liban5d (same soname), so torch's tensors, streams and
+        # the library share one HIP runtime.  Loaded after liban5d, it would
+        # be a second runtime in the process, with no device visible to torch.
+        if "torch" not in sys.modules:
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         L.an5d_last_error.restype = ctypes.c_char_p

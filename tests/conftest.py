@@ -15,7 +15,10 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def torch_cuda():
+    """torch with its GPU.  Only tests marked `gpu` use it, so a torch that
+    sees no device is a failure, not a skip (it would hide the device-tensor
+    entry points)."""
     import torch
     if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+        pytest.fail("torch sees no GPU in this process")
     return torch

@@ -135,8 +135,11 @@ def test_autograd_function_gradcheck(torch_cuda):
         mask[r:-r, r:-r] = 1.0
         fn = lambda u, cc: sweep(u, cc, name, n, T, div) * mask  # noqa: E731
         print(f"case gradcheck {name}", flush=True)
-        # the halo is a boundary condition: perturb only interior cells
-        assert torch.autograd.gradcheck(lambda ui, cc: fn(u0.detach() * (1 - mask) + ui * mask, cc),
+        # the halo is a boundary condition: only interior cells reach the
+        # sweep from the perturbed input (gradcheck perturbs u0 in place, so
+        # the halo comes from a separate copy)
+        halo = u0.detach().clone() * (1 - mask)
+        assert torch.autograd.gradcheck(lambda ui, cc: fn(halo + ui * mask, cc),
                                         (u0, c), eps=1e-6, atol=1e-7, rtol=1e-5, nondet_tol=1e-10)
 
 

@@ -1,0 +1,94 @@
+"""Parity at the benchmark sizes: star3d1r 512^3 (FP32, FP64) and j2d5pt FP32
+16384^2, each with the configuration the benchmark runs (explicit flags, else
+the autotuner's table, else the defaults).
+
+- The first launches are compared with the oracle bit for bit (the oracle
+  finishes a few steps of these grids in seconds).
+- The full 1000-step horizon is checked through properties that need no
+  oracle run: the tuned sweep equals a sweep with another kernel shape and
+  depth bit for bit (every cell's arithmetic is fixed by the source order, so
+  the tiling must not show), and 1000 steps in one call equal 400 + 600.
+- FMA mode at full size stays within the stated tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pyan5d
+
+pytestmark = pytest.mark.gpu
+
+FULL = [("star3d1r", np.float32, [512, 512, 512]),
+        ("star3d1r", np.float64, [512, 512, 512]),
+        ("j2d5pt", np.float32, [16384, 16384])]
+
+
+def same(a, b):
+    return np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("name,dt,n", FULL)
+def test_first_launches_bitwise_vs_oracle(name, dt, n):
+    rc = pyan5d.resolve_config(name, n, dt)
+    bt = rc["bt"]
+    # two launches where that is cheap for the oracle, one for the 2D grid
+    T = bt if O.info(name)["dim"] == 2 else 2 * bt
+    print(f"case full-size oracle {name} {np.dtype(dt).name} n={n} T={T} cfg={rc}", flush=True)
+    A = O.init_grid(name, n[0], dt)
+    B = A.copy()
+    pyan5d.forward_host(name, A, n, T, bt=bt, variant=rc["variant"], sl=rc["sl"])
+    O.run(name, B, n, T)
+    assert same(A[T % 2], B[T % 2])
+
+
+@pytest.mark.parametrize("name,dt,n", FULL)
+def test_full_horizon_shape_and_split_invariance(name, dt, n, torch_cuda):
+    torch = torch_cuda
+    T = 1000
+    rc = pyan5d.resolve_config(name, n, dt)
+    dcode = 0 if dt == np.float32 else 1
+    vs = pyan5d.variants(name, dcode)
+    # another shape and depth: the first variant at bt 2 (or variant 1 if
+    # the tuned run already uses variant 0 at bt 2)
+    alt_v, alt_bt = 0, 2
+    if (rc["variant"], rc["bt"]) == (alt_v, alt_bt):
+        alt_v = 1 if len(vs) > 1 else 0
+        alt_bt = 3 if alt_v == 0 else 2
+    host = pyan5d.init_grid(name, n, dt, mode=1)
+    base = torch.from_numpy(host).cuda()
+    del host
+    out = {}
+    print(f"case full-horizon {name} {np.dtype(dt).name} tuned {rc}", flush=True)
+    t = base.clone()
+    pyan5d.forward_device(name, t, n, T, bt=rc["bt"], variant=rc["variant"], sl=rc["sl"])
+    torch.cuda.synchronize()
+    out["tuned"] = t[T % 2].clone()
+    print(f"case full-horizon {name} {np.dtype(dt).name} variant {alt_v} bt {alt_bt}", flush=True)
+    t.copy_(base)
+    pyan5d.forward_device(name, t, n, T, bt=alt_bt, variant=alt_v)
+    torch.cuda.synchronize()
+    assert torch.equal(t[T % 2].view(torch.uint8), out["tuned"].view(torch.uint8))
+    print(f"case full-horizon {name} {np.dtype(dt).name} 400 + 600", flush=True)
+    t.copy_(base)
+    pyan5d.forward_device(name, t, n, 400, bt=rc["bt"], variant=rc["variant"], sl=rc["sl"])
+    pyan5d.forward_device(name, t, n, 600, bt=rc["bt"], variant=rc["variant"], sl=rc["sl"])
+    torch.cuda.synchronize()
+    assert torch.equal(t[0].view(torch.uint8), out["tuned"].view(torch.uint8))
+    # the run did move the field (no stage or launch was skipped)
+    assert not torch.equal(out["tuned"], base[0])
+    del t, base, out
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("name,dt,n", [FULL[0], FULL[2]])
+def test_fast_mode_full_size_within_tolerance(name, dt, n):
+    rc = pyan5d.resolve_config(name, n, dt, mode=pyan5d.FAST)
+    T = rc["bt"] + 1  # one full launch and a remainder launch
+    print(f"case full-size fast {name} n={n} T={T} cfg={rc}", flush=True)
+    A = O.init_grid(name, n[0], dt)
+    B = A.copy()
+    pyan5d.forward_host(name, A, n, T, bt=rc["bt"], variant=rc["variant"], sl=rc["sl"], mode=pyan5d.FAST)
+    O.run(name, B, n, T)
+    got, ref = A[T % 2].astype(np.float64), B[T % 2].astype(np.float64)
+    rel = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), np.finfo(dt).tiny))
+    assert rel <= 1e-5, rel

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <atomic>
 #include <vector>
 
 namespace an5d {
@@ -13,6 +14,22 @@ constexpr int kMaxMidBT = 8;  // deepest forward block with intermediate-stage o
 // grids (both buffers) above this size stream their stores past the 256 MB
 // last-level cache
 constexpr size_t kStreamBytes = 512ull << 20;
+
+constexpr int kMaxDevices = 64;
+
+// hipFuncSetAttribute acts on the current device, so a kernel that needs more
+// than the default 64 KB of LDS opts in once per device.  `done` is that
+// kernel's own per-device flag array (static storage, zero-initialised).
+inline hipError_t lds_opt_in(const void* kernel, size_t lds, std::atomic<int>* done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  if (done[dev].load(std::memory_order_acquire)) return hipSuccess;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) done[dev].store(1, std::memory_order_release);
+  return e;
+}
 
 enum KernelKind { kSweep = 0, kDirect = 1 };
 enum DType { kF32 = 0, kF64 = 1 };

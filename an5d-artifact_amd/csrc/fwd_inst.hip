@@ -37,26 +37,26 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
 template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS = false>
 hipError_t launch_sweep(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
   if (lds > 65536) {
-    // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default once
-    static const hipError_t attr = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (attr != hipSuccess) return attr;
+    // gfx950 has 160 KB of LDS per CU
+    static std::atomic<int> done[kMaxDevices];
+    const hipError_t e = lds_opt_in(
+        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>), lds, done);
+    if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>), grid, block, lds, st,
                      to_args<T>(a));
   return hipGetLastError();
 }
 
-// Resident workgroups per CU (0 if the query fails); computed once.
+// Resident workgroups per CU (0 if the query fails); computed once, on the
+// first device that asks (every device is a gfx950 with the same resources).
 template <int BT, int CK, int CJ, int NW, bool EXACT>
 int occupancy_once() {
   static const int n = [] {
     const void* k = reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>);
     const size_t lds = sweep_lds_bytes<SID, T, CK, CJ, NW>(BT);
-    if (lds > 65536 &&
-        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-      return 0;
+    static std::atomic<int> done[kMaxDevices];
+    if (lds > 65536 && lds_opt_in(k, lds, done) != hipSuccess) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * NW, lds) != hipSuccess) return 0;
     return blocks;

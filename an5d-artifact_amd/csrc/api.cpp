@@ -135,9 +135,9 @@ int device_cus() {
 // leave most SIMDs idle); without a device to ask, about 8192 wave tiles.
 int default_sl(const Variant& v, int bt, const Geom& g, long long ntk, long long ntj, int mode) {
   const long long n0 = g.D0 - 2 * g.r;
-  const bool tiles3 = v.kind == kSweep && g.dim == 3;
+  const bool tiles3 = (v.kind == kSweep || v.kind == kPlane) && g.dim == 3;
   long long slots = 0;
-  if (v.kind == kSweep && bt >= 1 && bt <= kMaxBT && v.occ[bt]) {
+  if ((v.kind == kSweep || v.kind == kPlane) && bt >= 1 && bt <= kMaxBT && v.occ[bt]) {
     const int occ = v.occ[bt](mode);
     const int cus = occ > 0 ? device_cus() : 0;
     slots = (long long)occ * cus * (tiles3 ? 1 : v.nw);  // workgroups (3D) or waves (2D)
@@ -173,6 +173,21 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
     lp.grid = dim3((unsigned)cdiv(total, 256));
     a.ntk = a.ntj = a.nseg = a.sl = a.SK = a.SJ = 0;
     if (cdiv(total, 256) >= (1LL << 31)) { err = "grid too large"; return false; }
+    return true;
+  }
+  if (v.kind == kPlane) {
+    // tiles of nw*cj rows x 64*ck cells, halos staged, nothing recomputed
+    if (g.dim != 3) { err = "the plane kernel is 3D only"; return false; }
+    const long long TK = 64LL * v.ck, TJ = (long long)v.nw * v.cj;
+    const long long ntk = cdiv(nk, TK), ntj = cdiv(nj, TJ);
+    if (sl <= 0) sl = default_sl(v, bt, g, ntk, ntj, mode);
+    const long long nseg = cdiv(n0, sl);
+    a.ntk = (int)ntk; a.ntj = (int)ntj; a.nseg = (int)nseg; a.sl = sl;
+    a.SK = (int)TK; a.SJ = (int)TJ;
+    lp.block = dim3(64 * v.nw);
+    const long long blocks = ntk * ntj * nseg;
+    if (blocks >= (1LL << 31)) { err = "grid too large"; return false; }
+    lp.grid = dim3((unsigned)blocks);
     return true;
   }
   const long long SK = 64LL * v.ck - 2LL * bt * R;

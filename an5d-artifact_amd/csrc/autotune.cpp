@@ -114,6 +114,8 @@ double ops_per_update(const StencilRT& s, const Variant& v, int mode, bool has_d
     // DPP halo per row and the middle-dim exchange, per cell
     ops += 2.0 * s.rad / v.ck;
     if (s.dim == 3) ops += 2.0 * s.rad * 2.0 / v.cj;
+  } else if (v.kind == kPlane) {
+    ops += 2.0 * s.rad;  // the dim-0 queue shifts by one plane per step
   } else {
     ops += 0.5 * s.ntaps;  // address arithmetic of the direct kernel
   }
@@ -139,6 +141,10 @@ double model_gstencil(const an5d_problem* p, const Variant& v, int bt, int sl, i
     if (g.dim == 3) ovj = lp.args.ntj * (double)v.nw * v.cj / nj;
     ovi = (lp.args.nseg * (double)lp.args.sl + 2.0 * bt * g.r * lp.args.nseg) / n0;
     waves = (double)lp.args.ntk * lp.args.ntj * lp.args.nseg * (g.dim == 3 ? v.nw : 1);
+  } else if (v.kind == kPlane) {
+    ovk = lp.args.ntk * 64.0 * v.ck / nk;
+    ovj = lp.args.ntj * (double)v.nw * v.cj / nj;
+    waves = (double)lp.args.ntk * lp.args.ntj * lp.args.nseg * v.nw;
   } else {
     waves = cells / v.ck / 64.0;
   }

@@ -31,7 +31,7 @@ inline hipError_t lds_opt_in(const void* kernel, size_t lds, std::atomic<int>* d
   return e;
 }
 
-enum KernelKind { kSweep = 0, kDirect = 1 };
+enum KernelKind { kSweep = 0, kDirect = 1, kPlane = 2 };
 enum DType { kF32 = 0, kF64 = 1 };
 
 // dtype-erased SweepArgs<T>

@@ -2,6 +2,7 @@
 // Built once per pair with -DAN5D_SID=<stencil id> -DAN5D_F64=<0|1>, so the
 // 42 translation units compile in parallel.
 #include "engine.hpp"
+#include "plane.hpp"
 #include "sweep.hpp"
 #include "variants.hpp"
 
@@ -74,6 +75,32 @@ hipError_t launch_direct(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds,
   return hipGetLastError();
 }
 
+template <int CK, int CJ, int NW, bool EXACT>
+hipError_t launch_plane(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  // the kernel keeps in-plane offsets in 32 bits
+  if ((long long)a.D1 * a.s1 >= (1LL << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((plane_kernel<SID, T, CK, CJ, NW, EXACT>), grid, block, lds, st, to_args<T>(a));
+  return hipGetLastError();
+}
+
+template <int CK, int CJ, int NW, bool EXACT>
+int occupancy_plane_once() {
+  static const int n = [] {
+    const void* k = reinterpret_cast<const void*>(&plane_kernel<SID, T, CK, CJ, NW, EXACT>);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * NW,
+                                                     plane_lds_bytes<T, CK, CJ, NW, Def<SID>::rad>()) !=
+        hipSuccess)
+      return 0;
+    return blocks;
+  }();
+  return n;
+}
+template <int CK, int CJ, int NW>
+int occupancy_plane(int mode) {
+  return mode ? occupancy_plane_once<CK, CJ, NW, false>() : occupancy_plane_once<CK, CJ, NW, true>();
+}
+
 // The shape's btmax lowered until the exchange buffers fit gfx950's 160 KB of LDS.
 template <int W>
 constexpr Shape fitted_shape() {
@@ -125,6 +152,20 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   d.fn[1] = &launch_direct<direct_ck<SID, T>()>;
   d.fn_fast[1] = d.fn[1];
   out.push_back(d);
+  // radius 2-4 3D stars: one step per launch with the plane staged in LDS
+  if constexpr (plane_taps_ok<Def<SID>>() && Def<SID>::rad >= 2) {
+    constexpr int CK = 2, CJ = AN5D_F64 ? 1 : 2, NW = AN5D_F64 ? 16 : 8;
+    Variant pv{};
+    pv.stencil = SID;
+    pv.dtype = AN5D_F64 ? kF64 : kF32;
+    pv.kind = kPlane;
+    pv.ck = CK; pv.cj = CJ; pv.nw = NW; pv.btmax = 1;
+    pv.lds[1] = plane_lds_bytes<T, CK, CJ, NW, Def<SID>::rad>();
+    pv.fn[1] = &launch_plane<CK, CJ, NW, true>;
+    pv.fn_fast[1] = &launch_plane<CK, CJ, NW, false>;
+    pv.occ[1] = &occupancy_plane<CK, CJ, NW>;
+    out.push_back(pv);
+  }
 }
 
 }  // namespace an5d

@@ -79,7 +79,9 @@ typedef struct an5d_stencil_info {
 } an5d_stencil_info;
 
 typedef struct an5d_variant_info {
-  int kind;  /* 0: temporally blocked sweep, 1: one step per launch */
+  int kind;  /* 0: temporally blocked sweep, 1: one step per launch from global
+               memory, 2: one step per launch with the plane staged in LDS
+               (3D stars of radius 2-4) */
   int ck, cj, nw, btmax;
   long long lds_bytes;
 } an5d_variant_info;

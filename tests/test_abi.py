@@ -58,9 +58,16 @@ def test_every_stencil_has_kernels_for_both_dtypes(name):
     for dt in (0, 1):
         vs = pyan5d.variants(name, dt)
         assert vs, name
-        assert vs[-1]["kind"] == 1  # the one-step-per-launch kernel is always there
+        kinds = [v["kind"] for v in vs]
+        assert kinds.count(1) == 1  # the one-step-per-launch kernel is always there
+        # the LDS-staged one-step kernel exactly for the radius 2-4 3D stars
+        d = O.info(name)
+        is_star3 = name.startswith("star3d") and d["rad"] >= 2
+        assert kinds.count(2) == (1 if is_star3 else 0), (name, kinds)
         for v in vs:
             assert 1 <= v["btmax"] <= 16
+            if v["kind"] == 2:
+                assert v["btmax"] == 1 and v["lds_bytes"] <= 160 * 1024
 
 
 def test_argument_errors_do_not_touch_the_gpu():

@@ -176,6 +176,31 @@ def test_rectangular_and_tiny_grids():
         assert same(A[1], B[1])
 
 
+@pytest.mark.parametrize("name", ["star3d2r", "star3d3r", "star3d4r"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_plane_kernel_ragged_grids(name, dt):
+    """The LDS-staged one-step kernel (kind 2) on grids smaller than one tile,
+    ragged tile edges in both in-plane dims and segments shorter than 2R+1."""
+    vs = [v for v in pyan5d.variants(name, 0 if dt == np.float32 else 1) if v["kind"] == 2]
+    assert len(vs) == 1
+    v = vs[0]
+    for n, sl in [([1, 1, 1], 0), ([3, 17, 130], 1), ([9, 5, 300], 2), ([20, 33, 257], 7), ([13, 16, 128], 0)]:
+        A = rand_grid(name, n, dt, seed=sum(n))
+        B = A.copy()
+        T = 3
+        print(f"case plane {name} {np.dtype(dt).name} n={n} sl={sl}", flush=True)
+        pyan5d.forward_host(name, A, n, T, variant=v["index"], sl=sl)
+        O.run(name, B, n, T)
+        assert same(A[T % 2], B[T % 2])
+        C = rand_grid(name, n, dt, seed=sum(n))
+        D = C.copy()
+        pyan5d.forward_host(name, C, n, T, variant=v["index"], sl=sl, mode=pyan5d.FAST)
+        O.run(name, D, n, T)
+        got, ref = C[T % 2].astype(np.float64), D[T % 2].astype(np.float64)
+        rel = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), np.finfo(dt).tiny))
+        assert rel <= (1e-5 if dt == np.float32 else 1e-12), rel
+
+
 @pytest.mark.parametrize("name", ["j2d5pt", "j2d9pt", "star2d3r", "box2d2r", "gradient2d",
                                   "star3d1r", "box3d1r", "j3d27pt", "star3d4r"])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])

@@ -154,7 +154,10 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   out.push_back(d);
   // radius 2-4 3D stars: one step per launch with the plane staged in LDS
   if constexpr (plane_taps_ok<Def<SID>>() && Def<SID>::rad >= 2) {
-    constexpr int CK = 2, CJ = AN5D_F64 ? 1 : 2, NW = AN5D_F64 ? 16 : 8;
+    // 16-row x 128-cell tiles (64 cells for FP64 radius 4, which would spill
+    // at two cells per lane), at most 128 VGPRs
+    constexpr int CK = (AN5D_F64 && Def<SID>::rad >= 4) ? 1 : 2, CJ = AN5D_F64 ? 1 : 2,
+                  NW = AN5D_F64 ? 16 : 8;
     Variant pv{};
     pv.stencil = SID;
     pv.dtype = AN5D_F64 ? kF64 : kF32;

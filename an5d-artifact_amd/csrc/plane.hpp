@@ -7,9 +7,9 @@
 // workgroup owns a tile of TJ = NW*CJ rows x TK = 64*CK cells of every plane
 // of its dim-0 segment and nothing is recomputed:
 //   - the plane's tile plus its R-cell halo ((TJ+2R) x (TK+2R) values) is
-//     staged in LDS, double-buffered: the next plane's tile is loaded into
-//     registers while the current one is evaluated, then written to the
-//     other buffer, one barrier per plane;
+//     staged in LDS, double-buffered: the tiles of the next two planes are
+//     in flight in registers while the current one is evaluated, and the
+//     next one is then written to the other buffer, one barrier per plane;
 //   - a stencil tap along dim 0 reads a per-thread queue of the 2R+1 planes
 //     of the thread's own cells (a star has no dim-0 tap off its own column);
 //   - lane l owns the cells k0 + l + 64*c (c < CK), so every global access
@@ -132,22 +132,31 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
 #pragma unroll
       for (int cc = 0; cc < CK; cc++) qv[s][jj][cc] = base[qoff[jj][cc]];
   }
-  // in flight: tile of plane i+1 and own cells of plane i+R+1
-  T qn[CJ][CK];
-  {
-    const T* bt = plane_ptr(i0 + 1);
+  // two planes in flight: the tiles of planes i+1, i+2 and the own cells of
+  // planes i+R+1, i+R+2 (A serves even steps, B odd ones, so a buffer is
+  // refilled right after it is consumed and never copied)
+  T tvB[PER];
+  T qnA[CJ][CK], qnB[CJ][CK];
+  auto load_tile = [&](T (&dst)[PER], int p) {
+    const T* bp = plane_ptr(p);
 #pragma unroll
-    for (int m = 0; m < PER; m++) tv[m] = bt[toff[m]];
-    const T* bq = plane_ptr(i0 + R + 1);
+    for (int m = 0; m < PER; m++) dst[m] = bp[toff[m]];
+  };
+  auto load_own = [&](T (&dst)[CJ][CK], int p) {
+    const T* bp = plane_ptr(p);
 #pragma unroll
     for (int jj = 0; jj < CJ; jj++)
 #pragma unroll
-      for (int cc = 0; cc < CK; cc++) qn[jj][cc] = bq[qoff[jj][cc]];
-  }
+      for (int cc = 0; cc < CK; cc++) dst[jj][cc] = bp[qoff[jj][cc]];
+  };
+  load_tile(tv, i0 + 1);
+  load_tile(tvB, i0 + 2);
+  load_own(qnA, i0 + R + 1);
+  load_own(qnB, i0 + R + 2);
   __syncthreads();
 
   int buf = 0;
-  for (int i = i0; i < i1; i++) {
+  auto step = [&](int i, T (&tn)[PER], T (&qn)[CJ][CK]) {
     const T* tile = lds + buf * TILE;
     T* out = a.dst + (long long)i * a.s0;
     static_for<0, CJ>([&](auto jc) {
@@ -162,13 +171,13 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
         if (own[JJ][C]) out[qoff[JJ][C]] = v;
       });
     });
-    // next plane's tile into the other buffer: its last readers finished
-    // before the barrier that ended the previous iteration
+    // plane i+1's tile into the other buffer: its last readers finished
+    // before the barrier that ended the previous step
     T* nxt = lds + (buf ^ 1) * TILE;
 #pragma unroll
     for (int m = 0; m < PER; m++) {
       const int e = (int)threadIdx.x + m * NTH;
-      if (e < TILE) nxt[e] = tv[m];
+      if (e < TILE) nxt[e] = tn[m];
     }
 #pragma unroll
     for (int s = 0; s < Q - 1; s++)
@@ -180,18 +189,14 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
     for (int jj = 0; jj < CJ; jj++)
 #pragma unroll
       for (int cc = 0; cc < CK; cc++) qv[Q - 1][jj][cc] = qn[jj][cc];
-    {
-      const T* bt = plane_ptr(i + 2);
-#pragma unroll
-      for (int m = 0; m < PER; m++) tv[m] = bt[toff[m]];
-      const T* bq = plane_ptr(i + R + 2);
-#pragma unroll
-      for (int jj = 0; jj < CJ; jj++)
-#pragma unroll
-        for (int cc = 0; cc < CK; cc++) qn[jj][cc] = bq[qoff[jj][cc]];
-    }
+    load_tile(tn, i + 3);
+    load_own(qn, i + R + 3);
     __syncthreads();
     buf ^= 1;
+  };
+  for (int i = i0; i < i1; i += 2) {
+    step(i, tv, qnA);
+    if (i + 1 < i1) step(i + 1, tvB, qnB);  // uniform over the workgroup
   }
 }
 

@@ -64,7 +64,7 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  int b = blockIdx.x;
+  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int tk = b % a.ntk;
   b /= a.ntk;
   const int tj = b % a.ntj;

@@ -269,6 +269,17 @@ struct MemGet {
 template <int R>
 constexpr int default_unroll() { return R <= 2 ? 2 * R + 1 : 1; }
 
+// Workgroups are dispatched round-robin over gfx950's 8 XCDs, each with its
+// own L2.  Workgroup b of nb takes logical tile (b % 8) * (nb / 8) + b / 8, so
+// every XCD walks a contiguous range of tiles and neighbouring tiles, which
+// re-read each other's halos, share an L2.  A bijection on [0, nb): the last
+// nb % 8 workgroups keep their own index.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int kXcds = 8;
+  const int q = nb / kXcds;
+  return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
+}
+
 // Input planes in flight ahead of their use.  A depth that divides the unroll
 // factor rotates by renaming; an unroll of 1 keeps a two-deep queue (one move
 // per cell and plane).  At most ~40 VGPRs of queued cells.
@@ -359,7 +370,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 
   int tk, tj = 0, seg;
   if constexpr (DIM3) {
-    int b = blockIdx.x;
+    int b = xcd_tile(blockIdx.x, gridDim.x);
     tk = b % a.ntk;
     b /= a.ntk;
     tj = b % a.ntj;

@@ -42,6 +42,12 @@ WORKLOADS = {
     "j2d5pt-f32-4096": dict(stencil="j2d5pt", dtype=np.float32, n=[4096, 4096], T=100,
                             desc="j2d5pt FP32 4096^2 T=100, (5.1N+12.1W+15C+12.2E+12.3S)/118"),
 }
+# the high-order set: seeded coefficients uniform (0, 1] normalised to sum 1
+for _name, _n in (("box2d4r", [16384, 16384]), ("star3d4r", [512, 512, 512]), ("j3d27pt", [512, 512, 512])):
+    for _dt, _tag in ((np.float32, "f32"), (np.float64, "f64")):
+        WORKLOADS[f"{_name}-{_tag}"] = dict(
+            stencil=_name, dtype=_dt, n=_n, T=1000,
+            desc=f"{_name} {_tag.upper()} {'x'.join(map(str, _n))} T=1000, seeded normalised coefficients")
 
 
 def log(msg):

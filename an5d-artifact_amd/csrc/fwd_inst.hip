@@ -79,6 +79,12 @@ template <int CK, int CJ, int NW, bool EXACT>
 hipError_t launch_plane(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
   // the kernel keeps in-plane offsets in 32 bits
   if ((long long)a.D1 * a.s1 >= (1LL << 31)) return hipErrorInvalidValue;
+  if (lds > 65536) {
+    static std::atomic<int> done[kMaxDevices];
+    const hipError_t e =
+        lds_opt_in(reinterpret_cast<const void*>(&plane_kernel<SID, T, CK, CJ, NW, EXACT>), lds, done);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((plane_kernel<SID, T, CK, CJ, NW, EXACT>), grid, block, lds, st, to_args<T>(a));
   return hipGetLastError();
 }
@@ -87,11 +93,11 @@ template <int CK, int CJ, int NW, bool EXACT>
 int occupancy_plane_once() {
   static const int n = [] {
     const void* k = reinterpret_cast<const void*>(&plane_kernel<SID, T, CK, CJ, NW, EXACT>);
+    constexpr size_t lds = plane_lds_bytes<T, CK, CJ, NW, Def<SID>::rad>();
+    static std::atomic<int> done[kMaxDevices];
+    if (lds > 65536 && lds_opt_in(k, lds, done) != hipSuccess) return 0;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * NW,
-                                                     plane_lds_bytes<T, CK, CJ, NW, Def<SID>::rad>()) !=
-        hipSuccess)
-      return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * NW, lds) != hipSuccess) return 0;
     return blocks;
   }();
   return n;
@@ -154,10 +160,10 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   out.push_back(d);
   // radius 2-4 3D stars: one step per launch with the plane staged in LDS
   if constexpr (plane_taps_ok<Def<SID>>() && Def<SID>::rad >= 2) {
-    // 16-row x 128-cell tiles (64 cells for FP64 radius 4, which would spill
-    // at two cells per lane), at most 128 VGPRs
-    constexpr int CK = (AN5D_F64 && Def<SID>::rad >= 4) ? 1 : 2, CJ = AN5D_F64 ? 1 : 2,
-                  NW = AN5D_F64 ? 16 : 8;
+    // 16-row x 128-cell tiles, at most 128 VGPRs; the R+2 LDS tile slots
+    // take 42-157 KB
+    constexpr int CK = 2, CJ = AN5D_F64 ? 1 : 2, NW = AN5D_F64 ? 16 : 8;
+    static_assert(plane_lds_bytes<T, CK, CJ, NW, Def<SID>::rad>() <= 160 * 1024, "plane tiles exceed LDS");
     Variant pv{};
     pv.stencil = SID;
     pv.dtype = AN5D_F64 ? kF64 : kF32;

@@ -1,17 +1,21 @@
 // One step of a high-order 3D star per launch, streamed along dim 0 with the
-// in-plane taps staged in LDS.
+// planes staged in LDS.
 //
 // The register pipeline of sweep_kernel keeps 2R+1 planes of every stage in
 // VGPRs and recomputes R overlapping rows per tile edge; at radius 2-4 that
 // either spills (FP64) or leaves one or two waves per SIMD.  Here a
 // workgroup owns a tile of TJ = NW*CJ rows x TK = 64*CK cells of every plane
 // of its dim-0 segment and nothing is recomputed:
-//   - the plane's tile plus its R-cell halo ((TJ+2R) x (TK+2R) values) is
-//     staged in LDS, double-buffered: the tiles of the next two planes are
-//     in flight in registers while the current one is evaluated, and the
-//     next one is then written to the other buffer, one barrier per plane;
-//   - a stencil tap along dim 0 reads a per-thread queue of the 2R+1 planes
-//     of the thread's own cells (a star has no dim-0 tap off its own column);
+//   - each plane's tile plus its R-cell halo ((TJ+2R) x (TK+2R) values) is
+//     read from global memory once and kept in a ring of R+2 LDS slots: the
+//     output plane i reads its in-plane taps from tile i and its dim-0 taps
+//     i+1 .. i+R from the own cells of the next R tiles, while tile i+R+1 is
+//     written into the slot that tile i-1 left; one barrier per plane;
+//   - the dim-0 taps i-R .. i-1 come from a per-thread register queue of the
+//     own cells of the last R planes (a star has no dim-0 tap off its own
+//     column);
+//   - the tiles of the next two planes are in flight in registers while a
+//     plane is evaluated;
 //   - lane l owns the cells k0 + l + 64*c (c < CK), so every global access
 //     and every LDS read of a wave is 64 consecutive elements.
 // Taps are evaluated in source order by eval_point / eval_fast, as in the
@@ -33,19 +37,21 @@ constexpr bool plane_taps_ok() {
 
 template <typename T, int CK, int CJ, int NW, int R>
 constexpr size_t plane_lds_bytes() {
-  return 2ull * (NW * CJ + 2 * R) * (64 * CK + 2 * R) * sizeof(T);
+  return (size_t)(R + 2) * (NW * CJ + 2 * R) * (64 * CK + 2 * R) * sizeof(T);
 }
 
-// Input of own cell (JJ, C): dim-0 taps from the queue, in-plane taps from
-// the LDS tile (`t` points at the cell's own position in it).
+// Input of own cell (JJ, C) of output plane i: dim-0 taps i-R .. i-1 from
+// the queue, the rest from the LDS tiles.  `t[m]` points at the cell's own
+// position in the tile of plane i+m (m = 0..R).
 template <class D, typename T, int CJ, int CK, int LK, int JJ, int C>
 struct PlaneGet {
-  const T (&qv)[2 * D::rad + 1][CJ][CK];
-  const T* t;
+  const T (&qv)[D::rad][CJ][CK];
+  const T* const (&t)[D::rad + 1];
   template <int A0, int A1, int A2>
   __device__ __forceinline__ T at() const {
-    if constexpr (A1 == 0 && A2 == 0) return qv[D::rad + A0][JJ][C];
-    else return t[A1 * LK + A2];
+    if constexpr (A0 < 0) return qv[D::rad + A0][JJ][C];
+    else if constexpr (A0 > 0) return t[A0][0];
+    else return t[0][A1 * LK + A2];
   }
 };
 
@@ -60,7 +66,7 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
   constexpr int NTH = 64 * NW;
   constexpr int TILE = LJ * LK;
   constexpr int PER = (TILE + NTH - 1) / NTH;
-  constexpr int Q = 2 * R + 1;
+  constexpr int NB = R + 2;  // LDS ring slots
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -110,77 +116,72 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
 
   extern __shared__ __align__(16) unsigned char an5d_lds[];
   T* lds = reinterpret_cast<T*>(an5d_lds);
+  // slot of plane p >= i0
+  auto slot = [&](int p) { return lds + ((p - i0) % NB) * TILE; };
 
-  // prologue: tile of plane i0 into buffer 0, queue planes i0-R .. i0+R
-  T tv[PER];
-  {
-    const T* base = plane_ptr(i0);
-#pragma unroll
-    for (int m = 0; m < PER; m++) tv[m] = base[toff[m]];
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-      const int e = (int)threadIdx.x + m * NTH;
-      if (e < TILE) lds[e] = tv[m];
-    }
-  }
-  T qv[Q][CJ][CK];
-#pragma unroll
-  for (int s = 0; s < Q; s++) {
-    const T* base = plane_ptr(i0 - R + s);
-#pragma unroll
-    for (int jj = 0; jj < CJ; jj++)
-#pragma unroll
-      for (int cc = 0; cc < CK; cc++) qv[s][jj][cc] = base[qoff[jj][cc]];
-  }
-  // two planes in flight: the tiles of planes i+1, i+2 and the own cells of
-  // planes i+R+1, i+R+2 (A serves even steps, B odd ones, so a buffer is
-  // refilled right after it is consumed and never copied)
-  T tvB[PER];
-  T qnA[CJ][CK], qnB[CJ][CK];
   auto load_tile = [&](T (&dst)[PER], int p) {
     const T* bp = plane_ptr(p);
 #pragma unroll
     for (int m = 0; m < PER; m++) dst[m] = bp[toff[m]];
   };
-  auto load_own = [&](T (&dst)[CJ][CK], int p) {
-    const T* bp = plane_ptr(p);
+  auto store_tile = [&](T* s, const T (&src)[PER]) {
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+      const int e = (int)threadIdx.x + m * NTH;
+      if (e < TILE) s[e] = src[m];
+    }
+  };
+
+  // prologue: tiles of planes i0 .. i0+R into their slots, the queue with the
+  // own cells of planes i0-R .. i0-1, then tiles i0+R+1 and i0+R+2 in flight
+  T tvA[PER], tvB[PER];
+#pragma unroll
+  for (int m = 0; m <= R; m++) {
+    load_tile(tvA, i0 + m);
+    store_tile(slot(i0 + m), tvA);
+  }
+  T qv[R][CJ][CK];
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    const T* bp = plane_ptr(i0 - R + s);
 #pragma unroll
     for (int jj = 0; jj < CJ; jj++)
 #pragma unroll
-      for (int cc = 0; cc < CK; cc++) dst[jj][cc] = bp[qoff[jj][cc]];
-  };
-  load_tile(tv, i0 + 1);
-  load_tile(tvB, i0 + 2);
-  load_own(qnA, i0 + R + 1);
-  load_own(qnB, i0 + R + 2);
+      for (int cc = 0; cc < CK; cc++) qv[s][jj][cc] = bp[qoff[jj][cc]];
+  }
+  load_tile(tvA, i0 + R + 1);
+  load_tile(tvB, i0 + R + 2);
   __syncthreads();
 
-  int buf = 0;
-  auto step = [&](int i, T (&tn)[PER], T (&qn)[CJ][CK]) {
-    const T* tile = lds + buf * TILE;
+  // A serves even steps, B odd ones, so a register buffer is refilled right
+  // after it is consumed and never copied
+  auto step = [&](int i, T (&tn)[PER]) {
+    const T* t[R + 1];
+#pragma unroll
+    for (int m = 0; m <= R; m++) t[m] = slot(i + m);
     T* out = a.dst + (long long)i * a.s0;
+    T centre[CJ][CK];
     static_for<0, CJ>([&](auto jc) {
       constexpr int JJ = decltype(jc)::value;
       static_for<0, CK>([&](auto kc) {
         constexpr int C = decltype(kc)::value;
-        const T* t = tile + (R + w * CJ + JJ) * LK + R + lane + 64 * C;
-        PlaneGet<D, T, CJ, CK, LK, JJ, C> get{qv, t};
+        const int pos = (R + w * CJ + JJ) * LK + R + lane + 64 * C;
+        const T* tc[R + 1];
+#pragma unroll
+        for (int m = 0; m <= R; m++) tc[m] = t[m] + pos;
+        PlaneGet<D, T, CJ, CK, LK, JJ, C> get{qv, tc};
         T v;
         if constexpr (EXACT) v = eval_point<D, T, DM>(get, c, div, has_div, gscale);
         else v = eval_fast<D, T>(get, c);
         if (own[JJ][C]) out[qoff[JJ][C]] = v;
+        centre[JJ][C] = tc[0][0];
       });
     });
-    // plane i+1's tile into the other buffer: its last readers finished
+    // tile i+R+1 into the slot of tile i-1, whose last readers finished
     // before the barrier that ended the previous step
-    T* nxt = lds + (buf ^ 1) * TILE;
+    store_tile(slot(i + R + 1), tn);
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-      const int e = (int)threadIdx.x + m * NTH;
-      if (e < TILE) nxt[e] = tn[m];
-    }
-#pragma unroll
-    for (int s = 0; s < Q - 1; s++)
+    for (int s = 0; s < R - 1; s++)
 #pragma unroll
       for (int jj = 0; jj < CJ; jj++)
 #pragma unroll
@@ -188,15 +189,13 @@ __device__ __forceinline__ void plane_body(const SweepArgs<T>& a) {
 #pragma unroll
     for (int jj = 0; jj < CJ; jj++)
 #pragma unroll
-      for (int cc = 0; cc < CK; cc++) qv[Q - 1][jj][cc] = qn[jj][cc];
-    load_tile(tn, i + 3);
-    load_own(qn, i + R + 3);
+      for (int cc = 0; cc < CK; cc++) qv[R - 1][jj][cc] = centre[jj][cc];
+    load_tile(tn, i + R + 3);
     __syncthreads();
-    buf ^= 1;
   };
   for (int i = i0; i < i1; i += 2) {
-    step(i, tv, qnA);
-    if (i + 1 < i1) step(i + 1, tvB, qnB);  // uniform over the workgroup
+    step(i, tvA);
+    if (i + 1 < i1) step(i + 1, tvB);  // uniform over the workgroup
   }
 }
 

@@ -1,6 +1,7 @@
-"""Parity at the benchmark sizes: star3d1r 512^3 (FP32, FP64) and j2d5pt FP32
-16384^2, each with the configuration the benchmark runs (explicit flags, else
-the autotuner's table, else the defaults).
+"""Parity at the benchmark sizes: star3d1r 512^3 (FP32, FP64), j2d5pt FP32
+16384^2 and star3d4r 512^3 (FP32, FP64; the LDS-staged one-step kernel), each
+with the configuration the benchmark runs (explicit flags, else the
+autotuner's table, else the defaults).
 
 - The first launches are compared with the oracle bit for bit (the oracle
   finishes a few steps of these grids in seconds).
@@ -20,7 +21,9 @@ pytestmark = pytest.mark.gpu
 
 FULL = [("star3d1r", np.float32, [512, 512, 512]),
         ("star3d1r", np.float64, [512, 512, 512]),
-        ("j2d5pt", np.float32, [16384, 16384])]
+        ("j2d5pt", np.float32, [16384, 16384]),
+        ("star3d4r", np.float32, [512, 512, 512]),
+        ("star3d4r", np.float64, [512, 512, 512])]
 
 
 def same(a, b):
@@ -48,12 +51,11 @@ def test_full_horizon_shape_and_split_invariance(name, dt, n, torch_cuda):
     rc = pyan5d.resolve_config(name, n, dt)
     dcode = 0 if dt == np.float32 else 1
     vs = pyan5d.variants(name, dcode)
-    # another shape and depth: the first variant at bt 2 (or variant 1 if
-    # the tuned run already uses variant 0 at bt 2)
-    alt_v, alt_bt = 0, 2
-    if (rc["variant"], rc["bt"]) == (alt_v, alt_bt):
-        alt_v = 1 if len(vs) > 1 else 0
-        alt_bt = 3 if alt_v == 0 else 2
+    # another kernel: the first variant (the second if the tuned run uses
+    # the first) at bt 2 where it goes that deep
+    alt_v = 0 if rc["variant"] != 0 else 1
+    alt_bt = min(2, vs[alt_v]["btmax"])
+    assert (alt_v, alt_bt) != (rc["variant"], rc["bt"])
     host = pyan5d.init_grid(name, n, dt, mode=1)
     base = torch.from_numpy(host).cuda()
     del host

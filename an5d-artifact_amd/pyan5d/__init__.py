@@ -173,6 +173,59 @@ def _dtype_code(dt):
     raise ValueError(f"unsupported dtype {dt}")
 
 
+def grid_extents(name, n):
+    """Full extents (interior + halo) of one buffer for interior extents n."""
+    d = info(name)
+    if len(n) != d["dim"]:
+        raise ValueError(f"{name} is {d['dim']}D: n must have {d['dim']} extents, got {list(n)}")
+    if any(int(x) < 1 for x in n):
+        raise ValueError(f"interior extents must be >= 1, got {list(n)}")
+    return [int(x) + 2 * d["rad"] for x in n]
+
+
+def _check_host(A, name, n, buffers):
+    """A host numpy array holding `buffers` grids of the reference layout."""
+    if not isinstance(A, np.ndarray):
+        raise TypeError("expected a numpy array")
+    _dtype_code(A.dtype)
+    if not A.flags.c_contiguous:
+        raise ValueError("host array must be C-contiguous")
+    want = tuple(([buffers] if buffers else []) + grid_extents(name, n))
+    if A.shape != want:
+        raise ValueError(f"host array shape {A.shape}, expected {want}")
+
+
+def _check_device(t, what, shape=None, dtype=None, contiguous=False):
+    """A torch tensor on the current GPU, of the given shape and dtype, with
+    positive strides that do not overlap (unit last stride)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{what} must be a GPU tensor (got device {t.device})")
+    if t.device.index != torch.cuda.current_device():
+        raise ValueError(f"{what} is on {t.device}, the current device is cuda:{torch.cuda.current_device()}")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{what} must be {dtype}, got {t.dtype}")
+    if t.dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"{what} must be float32 or float64, got {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what} shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if contiguous and not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    st, sh = t.stride(), t.shape
+    if st[-1] != 1:
+        raise ValueError(f"{what}: last dimension must be contiguous")
+    for i in range(len(sh) - 1):
+        if st[i] < st[i + 1] * sh[i + 1]:
+            raise ValueError(f"{what}: strides {st} overlap or are not outermost-first")
+
+
+def _check_pair(t, name, n, contiguous=False):
+    """Both buffers of a grid: a [2, D0, (D1,) D2] device tensor."""
+    _check_device(t, "grid tensor", shape=[2] + grid_extents(name, n), contiguous=contiguous)
+
+
 def make_problem(name, n, T, dtype, coefs=None, divisor=-1.0):
     p = Problem()
     p.stencil = stencil_id(name)
@@ -182,7 +235,11 @@ def make_problem(name, n, T, dtype, coefs=None, divisor=-1.0):
     p.timesteps = int(T)
     keep = None
     if coefs is not None:
-        keep = np.ascontiguousarray(np.asarray(coefs, dtype=np.float64))
+        keep = np.ascontiguousarray(np.asarray(coefs, dtype=np.float64)).ravel()
+        d = info(name)
+        want = d["ntaps"] if d["linear"] else 2
+        if keep.size != want:
+            raise ValueError(f"{name} takes {want} coefficients, got {keep.size}")
         p.coefs = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     p.divisor = float(divisor)
     return p, keep
@@ -196,7 +253,7 @@ def make_config(bt=0, variant=-1, sl=0, mode=EXACT):
 
 def forward_host(name, A, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0, mode=EXACT):
     """GPU forward on a host numpy array A[2, ...] (reference layout), in place; returns ms."""
-    assert A.flags.c_contiguous
+    _check_host(A, name, n, 2)
     p, keep = make_problem(name, n, T, A.dtype, coefs, divisor)
     cfg = make_config(bt, variant, sl, mode)
     ms = ctypes.c_double()
@@ -207,7 +264,7 @@ def forward_host(name, A, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl=0
 
 def cpu_forward(name, A, n, T, coefs=None, divisor=-1.0, threads=0):
     """The library's CPU path (OpenMP loop) on a host array, in place; returns ms."""
-    assert A.flags.c_contiguous
+    _check_host(A, name, n, 2)
     p, keep = make_problem(name, n, T, A.dtype, coefs, divisor)
     ms = ctypes.c_double()
     _check(lib().an5d_cpu_forward(ctypes.byref(p), A.ctypes.data, int(threads), ctypes.byref(ms)))
@@ -217,6 +274,7 @@ def cpu_forward(name, A, n, T, coefs=None, divisor=-1.0, threads=0):
 
 def kernel_stencil(name, A, compsize, T, scop=True):
     """The drop-in entry (common.h:42 signature) on a host array; returns ms."""
+    _check_host(A, name, [compsize] * info(name)["dim"], 2)
     ms = ctypes.c_double()
     _check(lib().an5d_kernel_stencil(stencil_id(name), _dtype_code(A.dtype), A.ctypes.data,
                                      int(compsize), int(T), 1 if scop else 0, ctypes.byref(ms)))
@@ -258,6 +316,7 @@ def forward_device(name, t, n, T, coefs=None, divisor=-1.0, bt=0, variant=-1, sl
                    mode=EXACT):
     """Enqueue T steps on a device tensor t[2, ...]; returns immediately."""
     d = info(name)
+    _check_pair(t, name, n)
     p, keep = make_problem(name, n, T, _np_dtype(t), coefs, divisor)
     cfg = make_config(bt, variant, sl, mode)
     L = layout_of(t, d["dim"])
@@ -270,6 +329,7 @@ def launch(name, t, n, steps, c0=0, coefs=None, divisor=-1.0, bt=0, variant=-1, 
            mode=EXACT):
     """One launch of `steps` fused steps from buffer c0%2 to (c0+1)%2."""
     d = info(name)
+    _check_pair(t, name, n)
     p, keep = make_problem(name, n, steps, _np_dtype(t), coefs, divisor)
     cfg = make_config(bt if bt else steps, variant, sl, mode)
     L = layout_of(t, d["dim"])
@@ -293,6 +353,8 @@ def adjoint_host(name, u0, target, n, T, coefs=None, divisor=-1.0, bt=0, variant
     Returns (loss, grad, gcoef, ms_fwd, ms_total)."""
     u0 = np.ascontiguousarray(u0)
     target = np.ascontiguousarray(target, dtype=u0.dtype)
+    _check_host(u0, name, n, 0)
+    _check_host(target, name, n, 0)
     d = info(name)
     p, keep = make_problem(name, n, T, u0.dtype, coefs, divisor)
     cfg = make_config(bt, variant, sl)
@@ -321,10 +383,27 @@ class AdjointWorkspace:
         self.buf = torch.zeros(self.nbytes, dtype=torch.uint8, device="cuda")  # zeroed once
 
 
+def _check_adjoint_args(name, n, u0, target, grad, gcoef, loss):
+    import torch
+    ext = grid_extents(name, n)
+    _check_device(u0, "u0", shape=ext, contiguous=True)
+    _check_device(target, "target", shape=ext, dtype=u0.dtype, contiguous=True)
+    _check_device(grad, "grad", shape=ext, dtype=u0.dtype, contiguous=True)
+    d = info(name)
+    ng = d["ntaps"] if d["linear"] else 2
+    _check_device(gcoef, "gcoef", shape=[ng], dtype=torch.float64, contiguous=True)
+    _check_device(loss, "loss", dtype=torch.float64, contiguous=True)
+    if loss.numel() < 1:
+        raise ValueError("loss needs one float64 element")
+
+
 def adjoint_device(name, u0, target, grad, gcoef, loss, n, T, ws, coefs=None, divisor=-1.0,
                    bt=0, variant=-1, sl=0, stream=None, time_forward=False):
     # ws must have been sized for the same bt
     """Device tensors: u0/target/grad single grids, gcoef/loss float64 tensors."""
+    _check_adjoint_args(name, n, u0, target, grad, gcoef, loss)
+    if not isinstance(ws, AdjointWorkspace):
+        raise TypeError("ws must be an AdjointWorkspace")
     p, keep = make_problem(name, n, T, _np_dtype(u0), coefs, divisor)
     cfg = make_config(bt, variant, sl)
     mf = ctypes.c_double()
@@ -348,9 +427,11 @@ class Batch:
         cfgs = (Config * n)()
         self._keep = []
         for i, (name, ext, T, t, coefs, divisor) in enumerate(items):
+            _check_pair(t, name, ext, contiguous=True)
             p, keep = make_problem(name, ext, T, _np_dtype(t), coefs, divisor)
             probs[i] = p
-            self._keep.append(keep)
+            # the captured graph holds the raw pointers: the tensors must outlive it
+            self._keep.append((keep, t))
             bufs[i] = t.data_ptr()
             cfgs[i] = make_config(bt, -1, sl)
         h = ctypes.c_void_p()
@@ -385,6 +466,9 @@ def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None
     gs = ctypes.c_double()
     ptr, lay = None, None
     if tensor is not None:
+        _check_pair(tensor, name, n)
+        if _np_dtype(tensor) != np.dtype(dtype):
+            raise ValueError("tensor dtype differs from dtype")
         ptr = ctypes.c_void_p(tensor.data_ptr())
         lay = ctypes.byref(layout_of(tensor, info(name)["dim"]))
     _check(lib().an5d_autotune(ctypes.byref(p), ptr, lay, int(mode), int(max_measured),

@@ -74,9 +74,11 @@ def test_argument_errors_do_not_touch_the_gpu():
     with pytest.raises(pyan5d.An5dError):
         pyan5d.stencil_id("nope")
     A = np.zeros((2, 6, 6), np.float32)
-    with pytest.raises(pyan5d.An5dError) as e:
-        pyan5d.forward_host("j2d5pt", A, [0, 4], 3)
-    assert e.value.code == -1
+    p, _ = pyan5d.make_problem("j2d5pt", [4, 4], 3, np.float32)
+    p.n[0] = 0
+    ms = ctypes.c_double()
+    assert pyan5d.lib().an5d_forward_host(ctypes.byref(p), A.ctypes.data, None, ctypes.byref(ms)) == -1
+    assert "extents" in pyan5d.lib().an5d_last_error().decode()
     with pytest.raises(pyan5d.An5dError):
         pyan5d.forward_host("j2d5pt", A, [4, 4], -1)
     p, _ = pyan5d.make_problem("j2d5pt", [4, 4], 3, np.float32)
@@ -98,3 +100,35 @@ def test_zero_steps_is_a_no_op():
     assert np.array_equal(A, B)
     assert pyan5d.kernel_stencil("j2d5pt", A, 4, 0, scop=True) == 0.0
     assert np.array_equal(A, B)
+
+
+def test_wrappers_reject_bad_tensors_before_any_launch():
+    """CPU tensors, single-buffer tensors, wrong extents, wrong dtypes and
+    wrongly sized gradient outputs raise in Python, before the library sees a
+    pointer."""
+    import torch
+    A = np.zeros((2, 6, 6), np.float32)
+    with pytest.raises(ValueError):
+        pyan5d.forward_host("j2d5pt", A, [0, 4], 3)
+    with pytest.raises(ValueError):
+        pyan5d.forward_host("j2d5pt", A[:1].copy(), [4, 4], 3)       # one buffer
+    with pytest.raises(ValueError):
+        pyan5d.forward_host("j2d5pt", A, [4, 5], 3)                  # wrong extents
+    with pytest.raises(ValueError):
+        pyan5d.forward_host("j2d5pt", A, [4, 4], 3, coefs=[1.0, 2.0])  # wrong tap count
+    with pytest.raises(ValueError):
+        pyan5d.forward_host("j2d5pt", A.astype(np.float16), [4, 4], 3)  # unsupported dtype
+    t = torch.zeros(2, 6, 6)
+    with pytest.raises(ValueError):
+        pyan5d.forward_device("j2d5pt", t, [4, 4], 3)                # CPU tensor
+    with pytest.raises(ValueError):
+        pyan5d.launch("j2d5pt", t, [4, 4], 1)
+    with pytest.raises((TypeError, ValueError)):
+        pyan5d.forward_device("j2d5pt", A, [4, 4], 3)                # not a tensor
+    u = torch.zeros(6, 6)
+    gc = torch.zeros(5, dtype=torch.float64)
+    loss = torch.zeros(1, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        pyan5d.adjoint_device("j2d5pt", u, u, u, gc, loss, [4, 4], 3, None)
+    with pytest.raises(ValueError):
+        pyan5d.Batch([("j2d5pt", [4, 4], 3, t, None, -1.0)])

@@ -41,9 +41,8 @@ def test_batch_matches_single_grid(torch_cuda):
     b.close()
 
 
-def test_batch_rejects_unsupported():
-    import torch
-    t = torch.zeros(2, 10, 10)
+def test_batch_rejects_unsupported(torch_cuda):
+    t = torch_cuda.zeros(2, 10, 10, device="cuda")
     with pytest.raises(pyan5d.An5dError):
         pyan5d.Batch([("box2d4r", [2, 2], 3, t, None, -1.0)])
 

@@ -2,6 +2,7 @@
 // No counterpart in the reference (it computes no gradients).
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -235,27 +236,20 @@ extern "C" int an5d_adjoint_workspace(const an5d_problem* p, const an5d_config* 
   return AN5D_OK;
 }
 
-extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const void* dtarget,
-                                   void* dgrad, double* dgcoef, double* dloss, void* ws,
-                                   size_t ws_bytes, const an5d_config* cfg, void* stream_v,
-                                   double* ms_fwd) {
-  AdjPlan ap;
-  int rc = plan_adjoint(p, cfg, ap);
-  if (rc) return rc;
-  if (!du0 || !dtarget || !dgrad || !dgcoef || !dloss || !ws)
-    return fail(AN5D_E_INVALID, "null device pointer");
-  if (ws_bytes < ap.ws_bytes) return fail(AN5D_E_INVALID, "workspace too small");
-  hipStream_t st = static_cast<hipStream_t>(stream_v);
+namespace an5d {
+
+// Enqueues the checkpointed forward pass, the seed, the reverse blocks and the
+// gradient reduction of plan `ap` on `st`.  `ws` holds ap.ws_bytes of
+// workspace whose grid padding is zero; `dcoef` is the device coefficient
+// table built by build_coefs (has_div, gscale from the same call).
+static int adjoint_enqueue(const AdjPlan& ap, const an5d_problem* p, const void* du0,
+                           const void* dtarget, void* dgrad, double* dgcoef, double* dloss, void* ws,
+                           const void* dcoef, int has_div, double gscale, hipStream_t st,
+                           double* ms_fwd) {
   const Geom& g = ap.g;
   const int R = g.r;
-  std::vector<unsigned char> cbytes;
-  int has_div = 0;
-  double gscale = 1.0;
   std::string err;
-  if (!build_coefs(p, cbytes, has_div, gscale, err)) return fail(AN5D_E_INVALID, err);
-  const void* dcoef = nullptr;
-  rc = device_coefs(cbytes, &dcoef);
-  if (rc) return rc;
+  int rc = AN5D_OK;
   const int L = (int)ap.blocks.size();  // checkpoint slot L holds u_T
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   char* base = static_cast<char*>(ws);
@@ -362,6 +356,130 @@ extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const
     (void)hipEventDestroy(ev1);
   }
   return AN5D_OK;
+}
+
+}  // namespace an5d
+
+extern "C" int an5d_adjoint_device(const an5d_problem* p, const void* du0, const void* dtarget,
+                                   void* dgrad, double* dgcoef, double* dloss, void* ws,
+                                   size_t ws_bytes, const an5d_config* cfg, void* stream_v,
+                                   double* ms_fwd) {
+  AdjPlan ap;
+  int rc = plan_adjoint(p, cfg, ap);
+  if (rc) return rc;
+  if (!du0 || !dtarget || !dgrad || !dgcoef || !dloss || !ws)
+    return fail(AN5D_E_INVALID, "null device pointer");
+  if (ws_bytes < ap.ws_bytes) return fail(AN5D_E_INVALID, "workspace too small");
+  std::vector<unsigned char> cbytes;
+  int has_div = 0;
+  double gscale = 1.0;
+  std::string err;
+  if (!build_coefs(p, cbytes, has_div, gscale, err)) return fail(AN5D_E_INVALID, err);
+  const void* dcoef = nullptr;
+  rc = device_coefs(cbytes, &dcoef);
+  if (rc) return rc;
+  return adjoint_enqueue(ap, p, du0, dtarget, dgrad, dgcoef, dloss, ws, dcoef, has_div, gscale,
+                         static_cast<hipStream_t>(stream_v), ms_fwd);
+}
+
+// ---- the adjoint without allocation ------------------------------------------
+// The workspace and the coefficient table live in `scratch`, a device array of
+// fixed size that every device holds from module load to process end.  In host
+// code `scratch` names a host-side stand-in, never device memory: the device
+// address comes from hipGetSymbolAddress, once per device, and only that
+// address is passed to kernels.
+namespace an5d {
+
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+constexpr size_t kNoAllocCoefBytes = 1024;  // >= (81 taps + 2) * 8 bytes
+static_assert(kNoAllocScratchBytes % 256 == 0, "scratch is carved in 256-byte units");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+// Device address of this device's `scratch` (looked up on the first call on
+// each device).
+static int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(AN5D_E_UNSUPPORTED, "device index beyond the library's table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(AN5D_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return AN5D_OK;
+}
+
+// Zeroes n16 16-byte words (a fill kernel, not a memset node).
+__global__ __launch_bounds__(256) void zero_words_kernel(uint4* __restrict__ p, long long n16) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// The coefficient table travels by value in the kernel arguments and is
+// written to device memory by the stream itself.
+struct CoefPack {
+  unsigned char bytes[kNoAllocCoefBytes];
+  int n;
+};
+__global__ __launch_bounds__(256) void stage_coefs_kernel(CoefPack pk, unsigned char* __restrict__ dst) {
+  for (int i = threadIdx.x; i < pk.n; i += blockDim.x) dst[i] = pk.bytes[i];
+}
+
+}  // namespace an5d
+
+extern "C" int an5d_adjoint_device_noalloc(const an5d_problem* p, const void* du0, const void* dtarget,
+                                           void* dgrad, double* dgcoef, double* dloss,
+                                           const an5d_config* cfg, void* stream_v) {
+  AdjPlan ap;
+  int rc = plan_adjoint(p, cfg, ap);
+  if (rc) return rc;
+  if (!du0 || !dtarget || !dgrad || !dgcoef || !dloss) return fail(AN5D_E_INVALID, "null device pointer");
+  const size_t need = ap.ws_bytes + kNoAllocCoefBytes;
+  if (need > kNoAllocScratchBytes)
+    return fail(AN5D_E_INVALID, "problem needs " + std::to_string(need) + " bytes of adjoint workspace; " +
+                                    "an5d_adjoint_device_noalloc holds at most " +
+                                    std::to_string(kNoAllocScratchBytes) +
+                                    " (use an5d_adjoint_device with a caller workspace)");
+  std::vector<unsigned char> cbytes;
+  int has_div = 0;
+  double gscale = 1.0;
+  std::string err;
+  if (!build_coefs(p, cbytes, has_div, gscale, err)) return fail(AN5D_E_INVALID, err);
+  if (cbytes.size() > kNoAllocCoefBytes) return fail(AN5D_E_INVALID, "coefficient table too large");
+  void* ptr = nullptr;
+  rc = scratch_address(&ptr);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream_v);
+  unsigned char* base = static_cast<unsigned char*>(ptr);
+  // the previous call (any geometry) left its data here: the grid padding and
+  // the partial-sum slots must start at zero
+  const long long n16 = (long long)(ap.ws_bytes / 16);
+  long long blocks = (n16 + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                     reinterpret_cast<uint4*>(base), n16);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "scratch clear");
+  CoefPack pk{};
+  memcpy(pk.bytes, cbytes.data(), cbytes.size());
+  pk.n = (int)cbytes.size();
+  unsigned char* dcoef = base + ap.ws_bytes;  // ws_bytes is a multiple of 256
+  hipLaunchKernelGGL(stage_coefs_kernel, dim3(1), dim3(256), 0, st, pk, dcoef);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "coefficient staging");
+  return adjoint_enqueue(ap, p, du0, dtarget, dgrad, dgcoef, dloss, base, dcoef, has_div, gscale, st,
+                         nullptr);
 }
 
 extern "C" int an5d_adjoint_host(const an5d_problem* p, const void* u0, const void* target,

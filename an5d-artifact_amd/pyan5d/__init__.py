@@ -96,6 +96,9 @@ def lib():
         L.an5d_adjoint_device.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                           P(Config), ctypes.c_void_p, P(ctypes.c_double)]
+        L.an5d_adjoint_device_noalloc.argtypes = [P(Problem), ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                  P(Config), ctypes.c_void_p]
         L.an5d_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Problem), P(ctypes.c_void_p),
                                         P(Config)]
         L.an5d_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -414,6 +417,22 @@ def adjoint_device(name, u0, target, grad, gcoef, loss, n, T, ws, coefs=None, di
                                      _stream_ptr(stream), ctypes.byref(mf) if time_forward else None))
     del keep
     return mf.value if time_forward else None
+
+
+def adjoint_device_noalloc(name, u0, target, grad, gcoef, loss, n, T, coefs=None, divisor=-1.0,
+                           bt=0, variant=-1, sl=0, stream=None):
+    """adjoint_device without a workspace: the library's fixed device scratch
+    array holds it (no allocation; problems beyond its size are rejected)."""
+    _check_adjoint_args(name, n, u0, target, grad, gcoef, loss)
+    p, keep = make_problem(name, n, T, _np_dtype(u0), coefs, divisor)
+    cfg = make_config(bt, variant, sl)
+    _check(lib().an5d_adjoint_device_noalloc(ctypes.byref(p), ctypes.c_void_p(u0.data_ptr()),
+                                             ctypes.c_void_p(target.data_ptr()),
+                                             ctypes.c_void_p(grad.data_ptr()),
+                                             ctypes.c_void_p(gcoef.data_ptr()),
+                                             ctypes.c_void_p(loss.data_ptr()), ctypes.byref(cfg),
+                                             _stream_ptr(stream)))
+    del keep
 
 
 class Batch:

@@ -172,6 +172,20 @@ int an5d_adjoint_workspace(const an5d_problem* prob, const an5d_config* cfg, siz
 int an5d_adjoint_device(const an5d_problem* prob, const void* u0, const void* target, void* grad,
                         double* gcoef, double* loss, void* workspace, size_t workspace_bytes,
                         const an5d_config* cfg, void* stream, double* ms_fwd);
+/* The same without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The workspace and the
+ * coefficient table live in a fixed 64 MiB device array that the library
+ * holds on every device it runs on until the process ends; problems whose
+ * an5d_adjoint_workspace() size exceeds 64 MiB - 1 KiB are rejected with
+ * AN5D_E_INVALID.  Each call clears the part it uses with a fill kernel (no
+ * memset) and passes the coefficients by value through a kernel argument, so
+ * the enqueued work reads no host memory.  All calls on one device share that
+ * array: calls that overlap on one device (other streams or threads) race.
+ * Results are bit for bit those of an5d_adjoint_device for the same inputs. */
+int an5d_adjoint_device_noalloc(const an5d_problem* prob, const void* u0, const void* target,
+                                void* grad, double* gcoef, double* loss, const an5d_config* cfg,
+                                void* stream);
 /* Host form: copies in, runs, copies out; *ms_total = forward + adjoint. */
 int an5d_adjoint_host(const an5d_problem* prob, const void* u0, const void* target, void* grad,
                       double* gcoef, double* loss, const an5d_config* cfg, double* ms_fwd,

@@ -84,8 +84,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int ID, typename T, int BT, int CK, int NW>
-__global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
+// One wave tile gw (< ntk * nseg) of a reverse launch; the caller skips
+// tiles past the end.
+template <int ID, typename T, int BT, int CK>
+__device__ __forceinline__ void adjoint_body(const AdjArgs<T>& a, int gw) {
   using D = Def<ID>;
   static_assert(D::dim == 2, "adjoint sweep is 2D");
   constexpr int R = D::rad;
@@ -105,9 +107,6 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
   constexpr int LFMAX = LF(0);
 
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int gw = blockIdx.x * NW + w;
-  if (gw >= a.ntk * a.nseg) return;
   const int tk = gw % a.ntk, seg = gw / a.ntk;
   const int kv0 = R + tk * a.SK;
   const int kl = kv0 - HA + lane * CK;
@@ -383,6 +382,14 @@ __global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
     // reduction sums the slots in a fixed order
     if (lane == 0) a.gslot[(size_t)gw * NG + G] += v;
   });
+}
+
+template <int ID, typename T, int BT, int CK, int NW>
+__global__ __launch_bounds__(64 * NW) void adjoint_kernel(AdjArgs<T> a) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = blockIdx.x * NW + w;
+  if (gw >= a.ntk * a.nseg) return;
+  adjoint_body<ID, T, BT, CK>(a, gw);
 }
 
 // lambda_T = (u_T - target) on the interior, 0 on the halo; the loss partial

@@ -1,7 +1,24 @@
-// Batched forward sweep over many independent 2D grids: host side.
-// No counterpart in the reference, which runs one grid per binary.
+// Batched forward sweeps, and batched forward + adjoint, over many
+// independent 2D grids: host side.  No counterpart in the reference, which
+// runs one grid per binary.
+//
+// Every instance is a chain of levels (batch.hpp).  Forward batch: the
+// instance's launch schedule, ping-ponging in the caller's two buffers.
+// Forward + adjoint batch, in a padded workspace the batch owns: PREP, the
+// forward pass (one FWD launch per adjoint block), SEED, the reverse blocks
+// (ADJ), then GRAD + FINISH.  The forward pass keeps the state at every C-th
+// block boundary ("coarse" slots); with C = 1 that is every checkpoint the
+// reverse blocks read.  When the batch's workspaces would exceed the memory
+// budget, the largest instances take C ~ sqrt(blocks): each reverse segment
+// of C blocks first recomputes its C-1 inner checkpoints from its coarse slot
+// into C-1 "fine" slots (two-level checkpointing: one more forward pass, far
+// fewer grids).  Chains are placed on a common level axis so that the
+// estimated work per level is as even as possible, and the level launches
+// are captured into one HIP graph.
+#include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -13,160 +30,754 @@
 namespace an5d {
 
 // batch_inst.hip
-hipError_t batch_launch_level(const BatchRec* recs, const BatchUnit* units, int nunits, int* counter,
-                              int grid, hipStream_t st);
+hipError_t batch_launch_level(int kind, const BatchRec* recs, const BatchUnit* units, int nunits,
+                              int* counter, int grid, hipStream_t st);
 hipError_t batch_launch_reset(int* counters, int n, hipStream_t st);
+int batch_occupancy(int kind);
 int batch_kind_of(int stencil);
 int batch_max_bt();
+int batch_adj_max_bt(int kind);
 int batch_ck();
+int batch_adj_ck();
 
+namespace {
+
+constexpr int kRowsPerUnit = 8;  // PREP / SEED / GRAD rows per work unit
+// segment lengths the measured search tries; the gradient-slot capacity is
+// sized for the shortest
+constexpr int kSlCandidates[] = {32, 64, 128, 256};
+constexpr int kMinBatchSl = 32;
+constexpr int kDefaultSl = 64;
+
+long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+
+struct Inst {
+  Geom g;
+  int stencil = 0, kind = 0, has_div = 0, T = 0;
+  const void* coef = nullptr;
+  std::vector<int> steps;  // forward batch: launch schedule; adjoint batch: blocks in time order
+  int sl_fixed = 0;        // caller's sl (0: the batch-wide one)
+  int bt_fixed = 0;        // forward batch: caller's bt (0: the batch-wide policy)
+  int bt_prior = 1;        // forward batch: the model prior's bt
+  // adjoint batch: coarse stride C (blocks), coarse slots S = ceil(L / C);
+  // workspace slots: coarse 0..S-1, u_T at S, fine 1..C-1 at S+1..S+C-1,
+  // lambda at S+C and S+C+1
+  int C = 1, S = 0;
+  struct Step {
+    int type, k, src, dst, aux;  // slot indices (aux: lambda in); k = steps of FWD / ADJ
+  };
+  std::vector<Step> chain;
+  // forward batch
+  void* dA = nullptr;
+  // adjoint batch
+  const void* u0 = nullptr;
+  const void* target = nullptr;
+  void* grad = nullptr;
+  double* gcoef = nullptr;
+  double* loss = nullptr;
+  char* ws = nullptr;
+  long long pitch = 0;
+  size_t grid_bytes = 0;
+  double* gslot = nullptr;
+  double* lslot = nullptr;
+  long long nslots = 0;
+  int ng = 0;
+  int start = 0;  // first level of the chain
+  int len() const { return steps.empty() ? 0 : (int)steps.size(); }
+  int nslots_ws() const { return S + C + 2; }
+};
+
+// The adjoint chain of an instance with coarse stride C.
+void build_chain(Inst& I, int C) {
+  const int L = I.len();
+  I.C = C;
+  I.S = (L + C - 1) / C;
+  const int S = I.S;
+  auto pass1_slot = [&](int j) {  // where the forward pass leaves u at block boundary j
+    if (j == L) return S;
+    if (j % C == 0) return j / C;
+    return S + j % C;
+  };
+  I.chain.clear();
+  I.chain.push_back({kUnitPrep, 0, -1, 0, -1});
+  for (int j = 1; j <= L; j++) I.chain.push_back({kUnitFwd, I.steps[j - 1], pass1_slot(j - 1), pass1_slot(j), -1});
+  I.chain.push_back({kUnitSeed, 0, S, S + C, -1});
+  int lam = 0;  // lambda buffer holding the current adjoint state
+  for (int sg = S - 1; sg >= 0; sg--) {
+    const int b0 = sg * C, e = std::min(L, b0 + C);
+    auto rslot = [&](int j) { return j == b0 ? sg : S + (j - b0); };
+    // the last segment's inner states are still in the fine slots from the
+    // forward pass; the others are recomputed from their coarse slot
+    if (sg != S - 1)
+      for (int j = b0 + 1; j < e; j++) I.chain.push_back({kUnitFwd, I.steps[j - 1], rslot(j - 1), rslot(j), -1});
+    for (int bk = e - 1; bk >= b0; bk--) {
+      I.chain.push_back({kUnitAdj, I.steps[bk], rslot(bk), S + C + (lam ^ 1), S + C + lam});
+      lam ^= 1;
+    }
+  }
+  I.chain.push_back({kUnitGrad, 0, S + C + lam, -1, -1});
+}
+
+}  // namespace
 }  // namespace an5d
 
 using namespace an5d;
 
 struct an5d_batch {
+  bool adjoint = false;
   int n = 0;
   int levels = 0;
-  std::vector<BatchRec*> d_recs;
-  std::vector<BatchUnit*> d_units;
-  std::vector<int> nunits;
-  int* d_counters = nullptr;
+  int sl = kDefaultSl;
+  int bt_policy = 0;  // forward batch, instances without a caller bt: 0 the model prior, k > 0 bt = k
+  int counters_cap = 0;  // levels the counter array holds
+  std::vector<Inst> inst;
+  // level tables: the records of level l start at rec_off[l], its units at
+  // unit_off[l]: first the nfwd[l] units of the forward-class kernel, then the
+  // nadj[l] units of the adjoint kernel (record ids relative to rec_off[l])
+  BatchRec* d_recs = nullptr;
+  BatchUnit* d_units = nullptr;
+  std::vector<long long> rec_off, unit_off;
+  std::vector<int> nfwd, nadj;
+  int* d_counters = nullptr;  // two per level
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
   long long total_units = 0;
 };
 
-static void batch_free(an5d_batch* b) {
-  if (!b) return;
-  for (auto* p : b->d_recs) (void)hipFree(p);
-  for (auto* p : b->d_units) (void)hipFree(p);
-  (void)hipFree(b->d_counters);
+namespace an5d {
+namespace {
+
+int chain_len(const an5d_batch* b, const Inst& I) {
+  return b->adjoint ? (int)I.chain.size() : I.len();
+}
+
+int inst_sl(const an5d_batch* b, const Inst& I) { return I.sl_fixed > 0 ? I.sl_fixed : b->sl; }
+
+void fwd_tiling(const Inst& I, int k, int sl, BatchRec& r) {
+  const int R = I.g.r;
+  const long long nk = I.g.D2 - 2 * R, n0 = I.g.D0 - 2 * R;
+  const long long SK = 64LL * batch_ck() - 2LL * k * R;
+  const long long s = sl < n0 ? sl : n0;
+  r.SK = (int)SK;
+  r.ntk = (int)cdiv(nk, SK);
+  r.sl = (int)s;
+  r.nseg = (int)cdiv(n0, s);
+}
+
+// the adjoint sweep's tiling (adjoint.cpp adj_launch_plan): lane-aligned
+// give-up HA per side, SK owned cells per wave tile
+bool adj_tiling(const Inst& I, int k, int sl, BatchRec& r, std::string& err) {
+  const int R = I.g.r, W = R;  // linear stencils reach R into the forward states
+  const long long ck = batch_adj_ck();
+  const long long HA = ((2LL * k - 2) * R + W + ck - 1) / ck * ck;
+  const long long SK = 64LL * ck - 2 * HA;
+  if (SK <= 0) { err = "adjoint block too deep for the tile"; return false; }
+  const long long nk = I.g.D2 - 2 * R, n0 = I.g.D0 - 2 * R;
+  const long long HU = 2LL * (k - 1) * R + W, DLT = W + (long long)(k - 2) * R;
+  const long long ring = 2LL * W + (2LL * k - 2) * R + 2;
+  const long long lo = R - HU - (DLT > 0 ? DLT : 0);
+  const long long hi = (I.g.D0 - R) + (DLT > 0 ? DLT : 0) + (long long)k * R + 2 * ring;
+  if (lo < -kAdjRowPad || hi >= I.g.D0 + kAdjRowPad || 64LL * ck > kAdjPad) {
+    err = "adjoint block reaches beyond the workspace padding";
+    return false;
+  }
+  const long long s = sl < n0 ? sl : n0;
+  r.SK = (int)SK;
+  r.ntk = (int)cdiv(nk, SK);
+  r.sl = (int)s;
+  r.nseg = (int)cdiv(n0, s);
+  return true;
+}
+
+char* ckpt(const Inst& I, int j) {
+  return I.ws + (size_t)j * I.grid_bytes + (size_t)(kAdjRowPad * I.pitch + kAdjPad) * 4;
+}
+
+BatchRec base_rec(const Inst& I, int type) {
+  BatchRec r;
+  memset(&r, 0, sizeof(r));
+  r.type = type;
+  r.kind = I.kind;
+  r.has_div = I.has_div;
+  r.D0 = (int)I.g.D0;
+  r.D2 = (int)I.g.D2;
+  r.R = I.g.r;
+  r.coef = I.coef;
+  r.s0 = I.pitch;
+  r.s0_ref = I.g.D2;
+  r.rows = kRowsPerUnit;
+  return r;
+}
+
+// Estimated cost of local level m of an instance (cell-steps, the adjoint's
+// weighted by its extra work); used only to place chains.
+double level_work(const an5d_batch* b, const Inst& I, int m) {
+  const double cells = (double)(I.g.D0 - 2 * I.g.r) * (I.g.D2 - 2 * I.g.r);
+  if (!b->adjoint) return cells * I.steps[m];
+  const Inst::Step& st = I.chain[m];
+  if (st.type == kUnitFwd) return cells * st.k;
+  if (st.type == kUnitAdj) return 2.0 * cells * st.k;
+  return 0.3 * cells;
+}
+
+// The records of local level m of instance I (appended to recs).
+bool level_recs(const an5d_batch* b, const Inst& I, int m, std::vector<BatchRec>& recs, std::string& err) {
+  const int sl = inst_sl(b, I);
+  if (!b->adjoint) {
+    BatchRec r = base_rec(I, kUnitFwd);
+    const int k = I.steps[m];
+    const size_t gbytes = (size_t)I.g.area * 4;
+    r.src = static_cast<char*>(I.dA) + (size_t)(m % 2) * gbytes;
+    r.dst = static_cast<char*>(I.dA) + (size_t)((m + 1) % 2) * gbytes;
+    r.s0 = I.g.D2;
+    r.bt = k;
+    fwd_tiling(I, k, sl, r);
+    recs.push_back(r);
+    return true;
+  }
+  const Inst::Step& st = I.chain[m];
+  const int nchunks = (int)cdiv(I.g.D0, kRowsPerUnit);
+  BatchRec r = base_rec(I, st.type);
+  switch (st.type) {
+    case kUnitPrep:
+      r.src = I.u0;
+      r.dst = ckpt(I, 0);
+      r.nckpt = I.S + I.C;  // every state slot gets the halo
+      r.ckpt_stride = (long long)(I.grid_bytes / 4);
+      r.slots = I.gslot;
+      r.nslots = I.nslots;
+      r.ng = I.ng;
+      r.ntk = nchunks;
+      r.nseg = 1;
+      recs.push_back(r);
+      break;
+    case kUnitFwd:
+      r.src = ckpt(I, st.src);
+      r.dst = ckpt(I, st.dst);
+      r.bt = st.k;
+      fwd_tiling(I, st.k, sl, r);
+      recs.push_back(r);
+      break;
+    case kUnitSeed:
+      r.src = ckpt(I, st.src);
+      r.aux = I.target;
+      r.dst = ckpt(I, st.dst);
+      r.slots = I.lslot;
+      r.ntk = nchunks;
+      r.nseg = 1;
+      recs.push_back(r);
+      break;
+    case kUnitAdj:
+      r.src = ckpt(I, st.src);
+      r.aux = ckpt(I, st.aux);
+      r.dst = ckpt(I, st.dst);
+      r.slots = I.gslot;
+      r.bt = st.k;
+      if (!adj_tiling(I, st.k, sl, r, err)) return false;
+      if ((long long)r.ntk * r.nseg > I.nslots) {
+        err = "gradient slots too few for this segment length";
+        return false;
+      }
+      recs.push_back(r);
+      break;
+    default: {
+      BatchRec g = base_rec(I, kUnitGrad);
+      g.src = ckpt(I, st.src);  // lambda_0
+      g.dst = I.grad;
+      g.ntk = nchunks;
+      g.nseg = 1;
+      recs.push_back(g);
+      BatchRec f = base_rec(I, kUnitFinish);
+      f.slots = I.gslot;
+      f.nslots = I.nslots;
+      f.ng = I.ng;
+      f.lslots = I.lslot;
+      f.nl = nchunks;
+      f.out = I.gcoef;
+      f.out_loss = I.loss;
+      f.ntk = 1;
+      f.nseg = 1;
+      recs.push_back(f);
+    }
+  }
+  return true;
+}
+
+// Place every chain on the level axis: longest work first, each at the
+// offset that least raises the squared level loads.
+void place_chains(an5d_batch* b) {
+  int S = 0;
+  for (const Inst& I : b->inst) S = std::max(S, chain_len(b, I));
+  std::vector<int> order(b->inst.size());
+  std::vector<double> total(b->inst.size(), 0.0);
+  for (size_t i = 0; i < b->inst.size(); i++) {
+    order[i] = (int)i;
+    for (int m = 0; m < chain_len(b, b->inst[i]); m++) total[i] += level_work(b, b->inst[i], m);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return total[x] > total[y]; });
+  std::vector<double> load(S > 0 ? S : 1, 0.0);
+  for (int i : order) {
+    Inst& I = b->inst[i];
+    const int len = chain_len(b, I);
+    std::vector<double> w(len);
+    for (int m = 0; m < len; m++) w[m] = level_work(b, I, m);
+    int best = 0;
+    double best_c = -1.0;
+    for (int s = 0; s + len <= S; s++) {
+      double c = 0.0;  // sum_m load[s+m] * w[m]: the part of the squared loads that depends on s
+      for (int m = 0; m < len; m++) c += load[s + m] * w[m];
+      if (best_c < 0.0 || c < best_c) {
+        best_c = c;
+        best = s;
+      }
+    }
+    I.start = best;
+    for (int m = 0; m < len; m++) load[best + m] += w[m];
+  }
+  b->levels = S;
+}
+
+double rec_cost(const BatchRec& r) {
+  const double cells = (double)r.sl * 64.0 * batch_ck();
+  if (r.type == kUnitAdj) return 2.0 * cells * r.bt;
+  if (r.type == kUnitFwd) return cells * r.bt;
+  return 0.0;
+}
+
+void drop_graph(an5d_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// (Re)build and upload the level tables for the batch's current sl.
+int build_levels(an5d_batch* b) {
+  std::vector<BatchRec> recs;
+  std::vector<BatchUnit> units;
+  b->rec_off.assign(b->levels + 1, 0);
+  b->unit_off.assign(b->levels + 1, 0);
+  b->nfwd.assign(b->levels, 0);
+  b->nadj.assign(b->levels, 0);
+  std::string err;
+  for (int l = 0; l < b->levels; l++) {
+    b->rec_off[l] = (long long)recs.size();
+    b->unit_off[l] = (long long)units.size();
+    std::vector<BatchRec> lv;
+    for (const Inst& I : b->inst) {
+      const int m = l - I.start;
+      if (m < 0 || m >= chain_len(b, I)) continue;
+      if (!level_recs(b, I, m, lv, err)) return fail(AN5D_E_UNSUPPORTED, err);
+    }
+    // the most expensive records first: their tiles start before the cheap ones
+    std::stable_sort(lv.begin(), lv.end(), [](const BatchRec& x, const BatchRec& y) {
+      return rec_cost(x) > rec_cost(y);
+    });
+    for (int pass = 0; pass < 2; pass++) {  // forward-class units, then adjoint units
+      long long nu = 0;
+      for (const BatchRec& r : lv) {
+        if ((r.type == kUnitAdj) != (pass == 1)) continue;
+        const int rid = (int)(recs.size() - b->rec_off[l]);
+        recs.push_back(r);
+        const long long nt = (long long)r.ntk * r.nseg;
+        for (long long t = 0; t < nt; t++) units.push_back(BatchUnit{rid, (int)t});
+        nu += nt;
+      }
+      if (nu >= (1LL << 31)) return fail(AN5D_E_UNSUPPORTED, "too many work units in one level");
+      (pass == 0 ? b->nfwd : b->nadj)[l] = (int)nu;
+    }
+  }
+  b->rec_off[b->levels] = (long long)recs.size();
+  b->unit_off[b->levels] = (long long)units.size();
+  b->total_units = (long long)units.size();
+  drop_graph(b);
+  (void)hipFree(b->d_recs);
+  (void)hipFree(b->d_units);
+  b->d_recs = nullptr;
+  b->d_units = nullptr;
+  hipError_t e = hipMalloc(&b->d_recs, sizeof(BatchRec) * (recs.size() ? recs.size() : 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_units, sizeof(BatchUnit) * (units.size() ? units.size() : 1));
+  if (e == hipSuccess && !recs.empty())
+    e = hipMemcpy(b->d_recs, recs.data(), sizeof(BatchRec) * recs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !units.empty())
+    e = hipMemcpy(b->d_units, units.data(), sizeof(BatchUnit) * units.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(AN5D_E_ALLOC, std::string("batch tables: ") + hipGetErrorString(e));
+  return AN5D_OK;
+}
+
+void batch_free(an5d_batch* b) {
+  if (!b) return;
+  drop_graph(b);
+  (void)hipFree(b->d_recs);
+  (void)hipFree(b->d_units);
+  (void)hipFree(b->d_counters);
+  for (Inst& I : b->inst) (void)hipFree(I.ws);
   if (b->cap) (void)hipStreamDestroy(b->cap);
   delete b;
 }
 
-extern "C" int an5d_batch_create(an5d_batch** out, int n, const an5d_problem* probs,
-                                 void* const* dA, const an5d_config* cfgs) {
+// Common per-instance checks and coefficient tables.
+int init_inst(const an5d_problem* p, int i, Inst& I) {
+  std::string err;
+  if (!make_geom(p, I.g, err)) return fail(AN5D_E_INVALID, "instance " + std::to_string(i) + ": " + err);
+  if (I.g.dim != 2 || p->dtype != AN5D_FLOAT32)
+    return fail(AN5D_E_UNSUPPORTED, "batched sweeps take 2D FP32 problems");
+  if (I.g.D0 >= (1LL << 31) / 2 || I.g.D2 >= (1LL << 31) / 2)
+    return fail(AN5D_E_UNSUPPORTED, "instance " + std::to_string(i) + " too large for a batch");
+  I.stencil = p->stencil;
+  I.kind = batch_kind_of(p->stencil);
+  if (I.kind < 0)
+    return fail(AN5D_E_UNSUPPORTED, "stencil " + stencil_rt(p->stencil).name + " is not in the batched kernel");
+  if (p->timesteps < 0) return fail(AN5D_E_INVALID, "instance " + std::to_string(i) + ": timesteps < 0");
+  I.T = p->timesteps;
+  std::vector<unsigned char> cb;
+  double gs = 1.0;
+  if (!build_coefs(p, cb, I.has_div, gs, err)) return fail(AN5D_E_INVALID, err);
+  return device_coefs(cb, &I.coef);
+}
+
+// Forward batch, bt not given: the autotuner's model prior at the batch
+// kernel's shape (CK = 4, four independent waves) and the batch's default
+// segment length ranks bt = 1 .. kBatchMaxBT.
+int prior_bt(const an5d_problem* p) {
+  auto vs = variants_of(p->stencil, p->dtype);
+  const Variant* v = nullptr;
+  for (const Variant* x : vs)
+    if (x->kind == kSweep && x->ck == batch_ck() && x->nw == 4) v = x;
+  int best = 1;
+  double best_g = -1.0;
+  for (int bt = 1; bt <= batch_max_bt(); bt++) {
+    if (!v || bt > v->btmax) break;
+    const double gs = model_gstencil(p, *v, bt, kDefaultSl, AN5D_EXACT);
+    if (gs > best_g) {
+      best_g = gs;
+      best = bt;
+    }
+  }
+  return best;
+}
+
+// Forward batch: every instance's launch schedule under the batch's bt
+// policy, the chains placed, the counters sized.
+int plan_forward(an5d_batch* b) {
+  for (Inst& I : b->inst) {
+    int bt = I.bt_fixed > 0 ? I.bt_fixed : (b->bt_policy > 0 ? b->bt_policy : I.bt_prior);
+    if (bt > batch_max_bt()) bt = batch_max_bt();
+    const int nl = an5d_schedule(I.T, bt, nullptr, 0);
+    I.steps.resize(nl);
+    an5d_schedule(I.T, bt, I.steps.data(), nl);
+  }
+  place_chains(b);
+  if (b->levels > b->counters_cap) {
+    (void)hipFree(b->d_counters);
+    b->d_counters = nullptr;
+    hipError_t e = hipMalloc(&b->d_counters, sizeof(int) * 2 * (b->levels > 0 ? b->levels : 1));
+    if (e != hipSuccess) return fail(AN5D_E_ALLOC, "hipMalloc(batch counters)");
+    b->counters_cap = b->levels;
+  }
+  return AN5D_OK;
+}
+
+}  // namespace
+}  // namespace an5d
+
+extern "C" int an5d_batch_create(an5d_batch** out, int n, const an5d_problem* probs, void* const* dA,
+                                 const an5d_config* cfgs) {
   if (!out || n < 1 || !probs || !dA) return fail(AN5D_E_INVALID, "batch needs n >= 1 problems and buffers");
   *out = nullptr;
   an5d_batch* b = new an5d_batch();
   b->n = n;
-  struct Inst {
-    Geom g;
-    std::vector<int> steps;
-    const void* coef;
-    int has_div, kind, sl;
-  };
-  std::vector<Inst> inst(n);
-  std::string err;
+  b->inst.resize(n);
   for (int i = 0; i < n; i++) {
-    const an5d_problem* p = &probs[i];
-    Inst& I = inst[i];
-    if (!make_geom(p, I.g, err)) { batch_free(b); return fail(AN5D_E_INVALID, "instance " + std::to_string(i) + ": " + err); }
-    if (I.g.dim != 2 || p->dtype != AN5D_FLOAT32) {
-      batch_free(b);
-      return fail(AN5D_E_UNSUPPORTED, "batched sweeps take 2D FP32 problems");
-    }
-    I.kind = batch_kind_of(p->stencil);
-    if (I.kind < 0) {
-      batch_free(b);
-      return fail(AN5D_E_UNSUPPORTED, "stencil " + stencil_rt(p->stencil).name + " is not in the batched kernel");
-    }
-    if (!dA[i] || p->timesteps < 0) { batch_free(b); return fail(AN5D_E_INVALID, "bad buffer or timesteps"); }
-    std::vector<unsigned char> cb;
-    double gs = 1.0;
-    if (!build_coefs(p, cb, I.has_div, gs, err)) { batch_free(b); return fail(AN5D_E_INVALID, err); }
-    int rc = device_coefs(cb, &I.coef);
+    Inst& I = b->inst[i];
+    int rc = init_inst(&probs[i], i, I);
     if (rc) { batch_free(b); return rc; }
-    int bt = (cfgs && cfgs[i].bt > 0) ? cfgs[i].bt : batch_max_bt();
-    if (bt > batch_max_bt()) { batch_free(b); return fail(AN5D_E_UNSUPPORTED, "batched bt is at most 4"); }
-    I.sl = (cfgs && cfgs[i].sl > 0) ? cfgs[i].sl : 64;
-    int nl = an5d_schedule(p->timesteps, bt, nullptr, 0);
-    I.steps.resize(nl);
-    an5d_schedule(p->timesteps, bt, I.steps.data(), nl);
-    if (nl > b->levels) b->levels = nl;
-  }
-  hipError_t e = hipMalloc(&b->d_counters, sizeof(int) * (b->levels > 0 ? b->levels : 1));
-  if (e != hipSuccess) { batch_free(b); return fail(AN5D_E_ALLOC, "hipMalloc(batch counters)"); }
-  const int ck = batch_ck();
-  for (int l = 0; l < b->levels; l++) {
-    std::vector<BatchRec> recs;
-    std::vector<BatchUnit> units;
-    for (int i = 0; i < n; i++) {
-      Inst& I = inst[i];
-      if (l >= (int)I.steps.size()) continue;
-      const int k = I.steps[l];
-      const int R = I.g.r;
-      const long long nk = I.g.D2 - 2 * R, n0 = I.g.D0 - 2 * R;
-      const long long SK = 64LL * ck - 2LL * k * R;
-      const long long ntk = (nk + SK - 1) / SK;
-      const long long sl = I.sl < n0 ? I.sl : n0;
-      const long long nseg = (n0 + sl - 1) / sl;
-      const size_t gbytes = (size_t)I.g.area * 4;
-      BatchRec r;
-      r.src = static_cast<char*>(dA[i]) + (size_t)(l % 2) * gbytes;
-      r.dst = static_cast<char*>(dA[i]) + (size_t)((l + 1) % 2) * gbytes;
-      r.coef = I.coef;
-      r.has_div = I.has_div;
-      r.kind = I.kind;
-      r.bt = k;
-      r.D0 = (int)I.g.D0;
-      r.D2 = (int)I.g.D2;
-      r.s0 = I.g.D2;
-      r.ntk = (int)ntk;
-      r.nseg = (int)nseg;
-      r.sl = (int)sl;
-      r.SK = (int)SK;
-      const int rid = (int)recs.size();
-      recs.push_back(r);
-      for (long long t = 0; t < ntk * nseg; t++) units.push_back(BatchUnit{rid, (int)t});
+    if (!dA[i]) { batch_free(b); return fail(AN5D_E_INVALID, "null buffer"); }
+    I.dA = dA[i];
+    I.pitch = I.g.D2;
+    const int bt = (cfgs && cfgs[i].bt > 0) ? cfgs[i].bt : 0;
+    if (bt > batch_max_bt()) {
+      batch_free(b);
+      return fail(AN5D_E_UNSUPPORTED, "batched bt is at most " + std::to_string(batch_max_bt()));
     }
-    BatchRec* dr = nullptr;
-    BatchUnit* du = nullptr;
-    e = hipMalloc(&dr, sizeof(BatchRec) * recs.size());
-    if (e == hipSuccess) e = hipMalloc(&du, sizeof(BatchUnit) * units.size());
-    if (e == hipSuccess) e = hipMemcpy(dr, recs.data(), sizeof(BatchRec) * recs.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(du, units.data(), sizeof(BatchUnit) * units.size(), hipMemcpyHostToDevice);
-    b->d_recs.push_back(dr);
-    b->d_units.push_back(du);
-    b->nunits.push_back((int)units.size());
-    b->total_units += (long long)units.size();
-    if (e != hipSuccess) { batch_free(b); return fail(AN5D_E_ALLOC, "batch tables"); }
+    I.bt_fixed = bt;
+    I.bt_prior = bt > 0 ? bt : prior_bt(&probs[i]);
+    I.sl_fixed = (cfgs && cfgs[i].sl > 0) ? cfgs[i].sl : 0;
   }
+  int rc = plan_forward(b);
+  if (rc == AN5D_OK) rc = build_levels(b);
+  if (rc) { batch_free(b); return rc; }
   *out = b;
   return AN5D_OK;
 }
 
-// Enqueue every level on `stream`: the first call captures the reset kernel
-// and the level launches into a HIP graph, later calls replay it.
+// Switched off: the first run of a forward + adjoint batch stalled on the
+// GPU twice (no fault, no error; the second time with the level launches
+// captured on one stream), and the cause has not been found.  The entry point
+// refuses every batch until it is.
+static constexpr bool kBatchAdjointEnabled = false;
+
+extern "C" int an5d_batch_adjoint_create(an5d_batch** out, int n, const an5d_problem* probs,
+                                         const an5d_batch_grad* io, const an5d_config* cfgs) {
+  if (!out || n < 1 || !probs || !io) return fail(AN5D_E_INVALID, "batch needs n >= 1 problems and buffers");
+  *out = nullptr;
+  if (!kBatchAdjointEnabled)
+    return fail(AN5D_E_UNSUPPORTED, "batched forward + adjoint is switched off: its first GPU run stalled and "
+                                    "the cause is not yet found (use an5d_adjoint_device per instance)");
+  an5d_batch* b = new an5d_batch();
+  b->adjoint = true;
+  b->n = n;
+  b->inst.resize(n);
+  std::string err;
+  for (int i = 0; i < n; i++) {
+    Inst& I = b->inst[i];
+    int rc = init_inst(&probs[i], i, I);
+    if (rc) { batch_free(b); return rc; }
+    if (I.T < 1) { batch_free(b); return fail(AN5D_E_INVALID, "adjoint instances need timesteps >= 1"); }
+    const an5d_batch_grad& g = io[i];
+    if (!g.u0 || !g.target || !g.grad || !g.gcoef || !g.loss) {
+      batch_free(b);
+      return fail(AN5D_E_INVALID, "instance " + std::to_string(i) + ": null device pointer");
+    }
+    I.u0 = g.u0;
+    I.target = g.target;
+    I.grad = g.grad;
+    I.gcoef = g.gcoef;
+    I.loss = g.loss;
+    const int btmax = batch_adj_max_bt(I.kind);
+    const int bt = (cfgs && cfgs[i].bt > 0) ? cfgs[i].bt : btmax;
+    if (bt > btmax) {
+      batch_free(b);
+      return fail(AN5D_E_UNSUPPORTED, "batched adjoint bt for " + stencil_rt(I.stencil).name + " is at most " +
+                                          std::to_string(btmax));
+    }
+    I.sl_fixed = (cfgs && cfgs[i].sl > 0) ? cfgs[i].sl : 0;
+    for (int t = 0; t < I.T; t += bt) I.steps.push_back(I.T - t < bt ? I.T - t : bt);
+    I.ng = stencil_rt(I.stencil).ntaps;
+    // gradient slots: the most wave tiles any reverse block of this instance
+    // uses at the shortest segment length the batch may choose
+    I.nslots = 0;
+    for (int k : I.steps) {
+      BatchRec r;
+      if (!adj_tiling(I, k, I.sl_fixed > 0 ? I.sl_fixed : kMinBatchSl, r, err)) {
+        batch_free(b);
+        return fail(AN5D_E_UNSUPPORTED, "instance " + std::to_string(i) + ": " + err);
+      }
+      I.nslots = std::max(I.nslots, (long long)r.ntk * r.nseg);
+    }
+    I.pitch = (I.g.D2 + 2 * kAdjPad + 63) / 64 * 64;
+    I.grid_bytes = round256((size_t)(I.g.D0 + 2 * kAdjRowPad) * I.pitch * 4);
+    build_chain(I, 1);
+  }
+  // memory budget: AN5D_BATCH_MEM bytes, else half of the free device memory;
+  // the largest instances move to two-level checkpointing until the
+  // workspaces fit it
+  size_t budget = 0;
+  if (const char* env = getenv("AN5D_BATCH_MEM")) budget = (size_t)strtoull(env, nullptr, 10);
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    hipError_t e = hipMemGetInfo(&free_b, &total_b);
+    if (e != hipSuccess) { batch_free(b); return hip_fail(e, "hipMemGetInfo"); }
+    budget = free_b / 2;
+  }
+  auto extra = [&](const Inst& I) {
+    return round256((size_t)I.nslots * I.ng * sizeof(double)) +
+           round256((size_t)cdiv(I.g.D0, kRowsPerUnit) * sizeof(double));
+  };
+  auto ws_bytes = [&](const Inst& I) { return I.grid_bytes * (size_t)I.nslots_ws() + extra(I); };
+  size_t total = 0;
+  for (const Inst& I : b->inst) total += ws_bytes(I);
+  std::vector<int> bysize(n);
+  for (int i = 0; i < n; i++) bysize[i] = i;
+  std::stable_sort(bysize.begin(), bysize.end(),
+                   [&](int x, int y) { return ws_bytes(b->inst[x]) > ws_bytes(b->inst[y]); });
+  for (int i : bysize) {
+    if (total <= budget) break;
+    Inst& I = b->inst[i];
+    const int L = I.len();
+    int C = 1;
+    while ((long long)C * C < L) C++;
+    if (C <= 1) continue;
+    total -= ws_bytes(I);
+    build_chain(I, C);
+    total += ws_bytes(I);
+  }
+  // (a budget that two-level checkpointing cannot meet is only a target: the
+  // allocations below then report whether the workspaces fit)
+  for (int i = 0; i < n; i++) {
+    Inst& I = b->inst[i];
+    const size_t gslot_off = I.grid_bytes * (size_t)I.nslots_ws();
+    const size_t lslot_off = gslot_off + round256((size_t)I.nslots * I.ng * sizeof(double));
+    const size_t ws = ws_bytes(I);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, ws);
+    if (e != hipSuccess) {
+      batch_free(b);
+      return fail(AN5D_E_ALLOC, "hipMalloc(batch adjoint workspace, " + std::to_string(ws) + " bytes): " +
+                                    hipGetErrorString(e));
+    }
+    I.ws = static_cast<char*>(p);
+    // the grids' padding is read and never written: zero it once here
+    e = hipMemset(p, 0, ws);
+    if (e != hipSuccess) { batch_free(b); return hip_fail(e, "hipMemset(batch workspace)"); }
+    I.gslot = reinterpret_cast<double*>(I.ws + gslot_off);
+    I.lslot = reinterpret_cast<double*>(I.ws + lslot_off);
+  }
+  place_chains(b);
+  hipError_t e = hipMalloc(&b->d_counters, sizeof(int) * 2 * (b->levels > 0 ? b->levels : 1));
+  if (e != hipSuccess) { batch_free(b); return fail(AN5D_E_ALLOC, "hipMalloc(batch counters)"); }
+  b->counters_cap = b->levels;
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) { batch_free(b); return hip_fail(e, "batch workspace clear"); }
+  int rc = build_levels(b);
+  if (rc) { batch_free(b); return rc; }
+  *out = b;
+  return AN5D_OK;
+}
+
+// Enqueue every level on `stream`: the first call after a (re)build captures
+// the reset kernel and the level launches into a HIP graph, later calls
+// replay it.  Everything is captured on one stream: a level's adjoint kernel
+// follows its forward-class kernel in stream order.  (A second stream would
+// have to join the capture through hipStreamWaitEvent on an event recorded
+// inside it, which HIP documents only as a plain wait: the second stream
+// would wait outside the graph for an event that never completes.)
 extern "C" int an5d_batch_run(an5d_batch* b, void* stream) {
   if (!b) return fail(AN5D_E_INVALID, "null batch");
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e;
   if (!b->exec) {
-    e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    if (!b->cap) {
+      e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    }
+    const int kf = b->adjoint ? kLevelFwdMisc : kLevelFwd;
+    const int cus = device_cus();
+    auto resident = [&](int kind) {
+      const int occ = batch_occupancy(kind);
+      return occ > 0 && cus > 0 ? (long long)occ * cus : 2048LL;
+    };
+    const long long res_f = resident(kf), res_a = resident(kLevelAdj);
+    auto grid_of = [](long long units, long long res) {
+      long long g = cdiv(units, 4);  // four waves per workgroup, one unit each at first
+      if (g > res) g = res;
+      return (int)(g < 1 ? 1 : g);
+    };
     e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    hipError_t le = batch_launch_reset(b->d_counters, b->levels, b->cap);
+    hipError_t le = batch_launch_reset(b->d_counters, 2 * b->levels, b->cap);
     for (int l = 0; l < b->levels && le == hipSuccess; l++) {
-      const int waves = b->nunits[l];
-      int grid = (waves + 3) / 4;
-      if (grid > 2048) grid = 2048;
-      if (grid < 1) grid = 1;
-      le = batch_launch_level(b->d_recs[l], b->d_units[l], b->nunits[l], b->d_counters + l, grid, b->cap);
+      const BatchRec* recs = b->d_recs + b->rec_off[l];
+      const BatchUnit* uf = b->d_units + b->unit_off[l];
+      const BatchUnit* ua = uf + b->nfwd[l];
+      if (b->nfwd[l] > 0)
+        le = batch_launch_level(kf, recs, uf, b->nfwd[l], b->d_counters + 2 * l, grid_of(b->nfwd[l], res_f),
+                                b->cap);
+      if (le == hipSuccess && b->nadj[l] > 0)
+        le = batch_launch_level(kLevelAdj, recs, ua, b->nadj[l], b->d_counters + 2 * l + 1,
+                                grid_of(b->nadj[l], res_a), b->cap);
     }
     e = hipStreamEndCapture(b->cap, &b->graph);
-    if (le != hipSuccess) return hip_fail(le, "batch level launch");
+    if (le != hipSuccess) {
+      drop_graph(b);
+      return hip_fail(le, "batch level launch");
+    }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    if (e != hipSuccess) {
+      drop_graph(b);
+      return hip_fail(e, "hipGraphInstantiate");
+    }
   }
   e = hipGraphLaunch(b->exec, st);
   if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  return AN5D_OK;
+}
+
+// Measured choice of the batch-wide segment length: the candidates are timed
+// on the whole level sequence (each replay rewrites the same outputs from
+// the same inputs), the fastest is kept.
+extern "C" int an5d_batch_tune(an5d_batch* b, int reps, int* best_sl, double* best_ms) {
+  if (!b) return fail(AN5D_E_INVALID, "null batch");
+  if (reps < 1) reps = 1;
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  // a forward batch advances the caller's grids: keep a copy to restore
+  std::vector<void*> snap(b->adjoint ? 0 : b->inst.size(), nullptr);
+  auto release_snap = [&]() {
+    for (void* p : snap) (void)hipFree(p);
+  };
+  for (size_t i = 0; i < snap.size(); i++) {
+    const size_t bytes = (size_t)2 * b->inst[i].g.area * 4;
+    e = hipMalloc(&snap[i], bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(snap[i], b->inst[i].dA, bytes, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) {
+      release_snap();
+      (void)hipStreamDestroy(st);
+      return fail(AN5D_E_ALLOC, std::string("batch tuning snapshot: ") + hipGetErrorString(e));
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int best = b->sl, best_pol = b->bt_policy;
+  double best_t = -1.0;
+  int rc = AN5D_OK;
+  // forward batch: bt policy (the model prior per instance, or one bt for all)
+  // x segment length; the forward + adjoint batch keeps its blocks
+  std::vector<int> policies = {b->bt_policy};
+  if (!b->adjoint) policies = {0, 3, 4, 5};
+  for (int pol : policies)
+  for (int sl : kSlCandidates) {
+    b->sl = sl;
+    if (!b->adjoint && pol != b->bt_policy) {
+      b->bt_policy = pol;
+      rc = plan_forward(b);
+      if (rc) break;
+    }
+    rc = build_levels(b);
+    if (rc) break;
+    rc = an5d_batch_run(b, st);  // capture + warm-up
+    if (rc) break;
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && rc == AN5D_OK; r++) rc = an5d_batch_run(b, st);
+    (void)hipEventRecord(e1, st);
+    e = hipEventSynchronize(e1);
+    if (rc) break;
+    if (e != hipSuccess) { rc = hip_fail(e, "batch tuning run"); break; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double t = ms / reps;
+    if (best_t < 0.0 || t < best_t) {
+      best_t = t;
+      best = sl;
+      best_pol = b->bt_policy;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (size_t i = 0; i < snap.size(); i++) {
+    e = hipMemcpyAsync(b->inst[i].dA, snap[i], (size_t)2 * b->inst[i].g.area * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess && rc == AN5D_OK) rc = hip_fail(e, "batch tuning restore");
+  }
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess && rc == AN5D_OK) rc = hip_fail(e, "batch tuning");
+  release_snap();
+  (void)hipStreamDestroy(st);
+  if (rc) return rc;
+  b->sl = best;
+  if (!b->adjoint && best_pol != b->bt_policy) {
+    b->bt_policy = best_pol;
+    rc = plan_forward(b);
+    if (rc) return rc;
+  }
+  rc = build_levels(b);
+  if (rc) return rc;
+  if (best_sl) *best_sl = best;
+  if (best_ms) *best_ms = best_t;
   return AN5D_OK;
 }
 

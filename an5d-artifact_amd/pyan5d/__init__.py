@@ -54,6 +54,11 @@ class VariantInfo(ctypes.Structure):
                 ("nw", ctypes.c_int), ("btmax", ctypes.c_int), ("lds_bytes", ctypes.c_longlong)]
 
 
+class BatchGrad(ctypes.Structure):
+    _fields_ = [("u0", ctypes.c_void_p), ("target", ctypes.c_void_p), ("grad", ctypes.c_void_p),
+                ("gcoef", ctypes.c_void_p), ("loss", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -101,6 +106,9 @@ def lib():
                                                   P(Config), ctypes.c_void_p]
         L.an5d_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Problem), P(ctypes.c_void_p),
                                         P(Config)]
+        L.an5d_batch_adjoint_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Problem), P(BatchGrad),
+                                                P(Config)]
+        L.an5d_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double)]
         L.an5d_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.an5d_batch_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int), P(ctypes.c_longlong)]
         L.an5d_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -436,8 +444,9 @@ def adjoint_device_noalloc(name, u0, target, grad, gcoef, loss, n, T, coefs=None
 
 
 class Batch:
-    """Batched forward over many device grids (an5d_batch_*).  `items` is a
-    list of (name, n, T, tensor[2, ...], coefs, divisor); optional per-item bt."""
+    """Batched forward over many device grids (an5d_batch_create).  `items` is
+    a list of (name, n, T, tensor[2, ...], coefs, divisor); bt / sl apply to
+    every item (0: the model prior / the batch-wide segment length)."""
 
     def __init__(self, items, bt=0, sl=0):
         n = len(items)
@@ -456,12 +465,22 @@ class Batch:
         h = ctypes.c_void_p()
         _check(lib().an5d_batch_create(ctypes.byref(h), n, probs, bufs, cfgs))
         self.handle = h
+        self._info()
+
+    def _info(self):
         lv, un = ctypes.c_int(), ctypes.c_longlong()
-        _check(lib().an5d_batch_info(h, ctypes.byref(lv), ctypes.byref(un)))
+        _check(lib().an5d_batch_info(self.handle, ctypes.byref(lv), ctypes.byref(un)))
         self.levels, self.units = lv.value, un.value
 
     def run(self, stream=None):
         _check(lib().an5d_batch_run(self.handle, _stream_ptr(stream)))
+
+    def tune(self, reps=2):
+        """Measured choice of the batch-wide segment length; returns (sl, ms)."""
+        sl, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().an5d_batch_tune(self.handle, int(reps), ctypes.byref(sl), ctypes.byref(ms)))
+        self._info()
+        return sl.value, ms.value
 
     def close(self):
         if self.handle:
@@ -473,6 +492,33 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class BatchAdjoint(Batch):
+    """Batched forward + adjoint (an5d_batch_adjoint_create).  `items` is a
+    list of (name, n, T, u0, target, grad, gcoef, loss, coefs, divisor): u0,
+    target, grad single device grids (FP32), gcoef / loss float64 device
+    tensors of ntaps / 1 elements.  bt / sl apply to every item (0: the
+    deepest compiled block / the batch-wide segment length)."""
+
+    def __init__(self, items, bt=0, sl=0):
+        n = len(items)
+        probs = (Problem * n)()
+        io = (BatchGrad * n)()
+        cfgs = (Config * n)()
+        self._keep = []
+        for i, (name, ext, T, u0, target, grad, gcoef, loss, coefs, divisor) in enumerate(items):
+            _check_adjoint_args(name, ext, u0, target, grad, gcoef, loss)
+            p, keep = make_problem(name, ext, T, _np_dtype(u0), coefs, divisor)
+            probs[i] = p
+            self._keep.append((keep, u0, target, grad, gcoef, loss))
+            io[i].u0, io[i].target, io[i].grad = u0.data_ptr(), target.data_ptr(), grad.data_ptr()
+            io[i].gcoef, io[i].loss = gcoef.data_ptr(), loss.data_ptr()
+            cfgs[i] = make_config(bt, -1, sl)
+        h = ctypes.c_void_p()
+        _check(lib().an5d_batch_adjoint_create(ctypes.byref(h), n, probs, io, cfgs))
+        self.handle = h
+        self._info()
 
 
 def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None, timesteps=1):

@@ -191,18 +191,56 @@ int an5d_adjoint_host(const an5d_problem* prob, const void* u0, const void* targ
                       double* gcoef, double* loss, const an5d_config* cfg, double* ms_fwd,
                       double* ms_total);
 
-/* ---- batched forward (2D, FP32) ------------------------------------------
- * No counterpart in the reference (one grid per binary).  Advances n
- * independent grids (stencils j2d5pt, j2d9pt, star2d2r, j2d9pt-gol; any sizes
- * and timesteps) together: launch l of every instance's schedule goes into
- * one kernel whose waves pull (instance, tile) work units from a device
- * queue; the level sequence is captured into a HIP graph on the first run.
- * dA[i]: both buffers of instance i, reference layout.  After a run, buffer
- * T_i%2 of instance i holds its u_T, bit for bit the single-grid result.
- * cfgs (optional): per-instance bt (<= 4) and sl. */
+/* ---- batched forward and batched forward + adjoint (2D, FP32) -------------
+ * No counterpart in the reference (one grid per binary, one grid per launch:
+ * compiled/float/j2d5pt-256-10-128_host.cu:74-77).  Stencils j2d5pt, j2d9pt,
+ * star2d2r and j2d9pt-gol; any sizes and timesteps.
+ *
+ * Each instance's work is a chain of levels; one kernel launch covers one
+ * level of every instance, its waves pulling (instance, tile) work units of
+ * every kind from a device counter, and the whole level sequence is captured
+ * into one HIP graph on the first run (replayed afterwards).  Chains are
+ * placed on the level axis so that the estimated work per level is even.
+ *
+ * an5d_batch_create: forward only.  dA[i] holds both buffers of instance i
+ * (reference layout); after a run buffer T_i%2 holds u_T, bit for bit the
+ * single-grid result.  cfgs (optional): per-instance bt (<= 5; 0: ranked by
+ * the autotuner's model at the batch kernel's shape) and sl (0: batch-wide).
+ *
+ * an5d_batch_adjoint_create: forward + adjoint.  Currently switched off: it
+ * returns AN5D_E_UNSUPPORTED (its first GPU run stalled; cause not found).
+ * Designed behaviour: for every instance, the
+ * same loss, gradients and conventions as an5d_adjoint_device (single grids
+ * in the reference layout; grad's halo is 0), with the forward states
+ * checkpointed every bt steps in a workspace the batch allocates and owns
+ * ((T/bt + 3) padded grids per instance).  When the workspaces would exceed
+ * the memory budget ($AN5D_BATCH_MEM bytes, else half the free device
+ * memory), the largest instances keep only every C-th checkpoint (C ~
+ * sqrt(T/bt)) and recompute the others segment by segment (about sqrt(T/bt)
+ * grids, one more forward pass).  cfgs (optional): per-instance
+ * adjoint block bt (0: the deepest compiled: 5 for j2d5pt, 4 for j2d9pt-gol,
+ * 3 for the radius-2 stars) and sl.  Loss and gradient sums are reduced in a
+ * fixed order, so repeated runs give identical results.
+ *
+ * an5d_batch_tune: times the level sequence (`reps` replays each) at each
+ * candidate batch-wide segment length (32-256) and, for a forward batch, bt
+ * policy (each instance's model-prior bt, or bt = 3, 4, 5 for all instances
+ * without a caller bt), and keeps the fastest; a forward batch's grids are
+ * restored afterwards.  an5d_batch_run: enqueue one run on
+ * `stream` (a hipStream_t); runs that overlap on one batch race. */
 typedef struct an5d_batch an5d_batch;
+typedef struct an5d_batch_grad {
+  const void* u0;     /* initial state, one grid (halo included) */
+  const void* target; /* one grid */
+  void* grad;         /* out: dL/du0, one grid */
+  double* gcoef;      /* out: dL/dcoefs, one per tap */
+  double* loss;       /* out: L */
+} an5d_batch_grad;
 int an5d_batch_create(an5d_batch** out, int n, const an5d_problem* probs, void* const* dA,
                       const an5d_config* cfgs);
+int an5d_batch_adjoint_create(an5d_batch** out, int n, const an5d_problem* probs,
+                              const an5d_batch_grad* io, const an5d_config* cfgs);
+int an5d_batch_tune(an5d_batch* batch, int reps, int* best_sl, double* best_ms);
 int an5d_batch_run(an5d_batch* batch, void* stream);
 int an5d_batch_info(const an5d_batch* batch, int* levels, long long* work_units);
 int an5d_batch_destroy(an5d_batch* batch);

@@ -1,20 +1,25 @@
-// Benchmark binary: the reference's common.h harness (common.h:21-180)
-// restated over liban5d, with the same CLI (-s -t -n -c -h), the same
-// output lines (parsed by compiled/run.sh:29 and summarize.sh:32-42), and
-// extra lines after them: GStencil/s, a relative-error verdict and, for
-// float, the error against an FP64 CPU run of the same input.
+// Benchmark binary bin/<stencil>.<float|double>: the reference's a.out for one
+// stencil, over liban5d.
+//
+// The command line (-s -t -n -c -h), the error messages and exit codes, and
+// the output lines up to "Average:" and "Max Error:" / "RMS Error:" follow the
+// reference harness (common.h:21-30,85-180): compiled/run.sh:29 and
+// summarize.sh read them.  After them this harness prints its own lines: the
+// GStencil/s rate, a relative-error verdict, and for float runs the error
+// against an FP64 run of the CPU path on the same input.
 //
 // Built once per stencil x type:  -DAN5D_STENCIL="j2d5pt" -DSB_TYPE=float.
-// The stencil's dim / radius / FPP come from the library registry instead of
-// the BENCH_* macros.  Tiling can be forced with AN5D_BT, AN5D_VARIANT and
-// AN5D_SL in the environment (read by the library).
+// Dimension, radius and FLOPs per update come from the library's registry.
+// Tiling can be forced with AN5D_BT, AN5D_VARIANT, AN5D_SL and AN5D_MODE in
+// the environment (read by the library).
 #include <float.h>
 #include <getopt.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
-#include <time.h>
+
+#include <vector>
 
 #include "../../include/an5d.h"
 #include "../../include/an5d_kernel_stencil.h"
@@ -28,11 +33,27 @@
 
 AN5D_DEFINE_KERNEL_STENCIL(AN5D_STENCIL)
 
-static an5d_stencil_info g_info;
+namespace {
 
-static void usage(char* cmd) {
-  printf("%s [OPTION] ...\n", cmd);
-  printf("\n");
+constexpr int kDtype = sizeof(SB_TYPE) == 8 ? AN5D_FLOAT64 : AN5D_FLOAT32;
+
+struct Shape {
+  an5d_stencil_info info;
+  long long cells;     // per buffer, halo included
+  long long interior;  // updated cells per step
+};
+
+Shape shape_of(const an5d_stencil_info& info, int n) {
+  Shape s{info, 1, 1};
+  for (int d = 0; d < info.dim; d++) {
+    s.cells *= n + 2LL * info.rad;
+    s.interior *= n;
+  }
+  return s;
+}
+
+void print_usage(const char* cmd) {
+  printf("%s [OPTION] ...\n\n", cmd);
   printf("-s\tSpecify compute size\n");
   printf("-t\tSpecify the number of timesteps\n");
   printf("-n\tSpecify the number of execution\n");
@@ -40,171 +61,145 @@ static void usage(char* cmd) {
   printf("-h\tShow this usage\n");
 }
 
-// common.h:44-56
-template <typename T>
-static void init_grid(T* A, int compsize) {
-  size_t gridarea = 1, dimsize = compsize + g_info.rad * 2;
-  for (int i = 0; i < g_info.dim; i++) gridarea *= dimsize;
-#pragma omp parallel for
-  for (size_t i = 0; i < gridarea; i++) {
-    unsigned int seed = i + compsize;
-    A[0 * gridarea + i] = 1000 * (T)(rand_r(&seed)) / (T)(RAND_MAX);
-    A[1 * gridarea + i] = A[0 * gridarea + i];
+// Both buffers from the reference's generator (common.h:44-56: rand_r seeded
+// with cell index + n, scaled to [0, 1000]).
+void fill(SB_TYPE* A, const Shape& s, int n) {
+  if (an5d_init_grid(kDtype, A, s.cells, n, 0) != AN5D_OK) {
+    fprintf(stderr, "[AN5D ERROR] %s\n", an5d_last_error());
+    exit(EXIT_FAILURE);
   }
 }
 
-// common.h:58-78, plus a relative-error / non-finite check
-template <typename T>
-static bool check_result(const T* A, const T* B, int compsize, int timestep) {
-  int t = timestep % 2;
-  size_t gridarea = 1, dimsize = compsize + g_info.rad * 2;
-  for (int i = 0; i < g_info.dim; i++) gridarea *= dimsize;
-  double error_max = 0.0, rel_max = 0.0;
-  long double error_sum = 0.0;
-  long long nonfinite_mismatch = 0;
-  const double floor_ = sizeof(T) == 4 ? (double)FLT_MIN : DBL_MIN;
-#pragma omp parallel for reduction(max : error_max, rel_max) reduction(+ : error_sum, nonfinite_mismatch)
-  for (size_t i = 0; i < gridarea; i++) {
-    const T a = A[t * gridarea + i], b = B[t * gridarea + i];
-    double error = fabs((double)(a - b));
-    error_max = (error > error_max) ? error : error_max;
-    error_sum += error * error;
-    if (isfinite(a) && isfinite(b)) {
-      double rel = error / fmax(fabs((double)b), floor_);
-      rel_max = rel > rel_max ? rel : rel_max;
-    } else if (!(isnan(a) && isnan(b)) && a != b) {
-      nonfinite_mismatch++;
+// Buffer T%2 of the GPU run against the CPU run: the reference's two lines
+// (max and RMS absolute error, common.h:58-78), then the relative error with
+// the smallest normal as floor, non-finite mismatches and the verdict.
+bool compare(const SB_TYPE* gpu, const SB_TYPE* cpu, const Shape& s, int T) {
+  const SB_TYPE* g = gpu + (size_t)(T % 2) * s.cells;
+  const SB_TYPE* c = cpu + (size_t)(T % 2) * s.cells;
+  const double floor_rel = kDtype == AN5D_FLOAT64 ? DBL_MIN : (double)FLT_MIN;
+  double amax = 0.0, rmax = 0.0;
+  long double sq = 0.0;
+  long long bad = 0;
+#pragma omp parallel for reduction(max : amax, rmax) reduction(+ : sq, bad)
+  for (long long i = 0; i < s.cells; i++) {
+    const double d = fabs((double)(g[i] - c[i]));
+    if (d > amax) amax = d;
+    sq += (long double)d * d;
+    if (isfinite(g[i]) && isfinite(c[i])) {
+      const double r = d / fmax(fabs((double)c[i]), floor_rel);
+      if (r > rmax) rmax = r;
+    } else if (!(isnan(g[i]) && isnan(c[i])) && g[i] != c[i]) {
+      bad++;
     }
   }
-  printf("Max Error: %e\n", error_max);
-  printf("RMS Error: %Le\n", sqrtl(error_sum / gridarea));
-  const double rtol = sizeof(T) == 4 ? 1e-5 : 1e-12;
-  printf("Max Rel Error: %e (floor %e, non-finite mismatches %lld)\n", rel_max, floor_,
-         nonfinite_mismatch);
-  const bool pass = rel_max <= rtol && nonfinite_mismatch == 0;
-  printf("Verdict: %s (rtol %g)\n", pass ? "PASS" : "FAIL", rtol);
-  return pass;
+  printf("Max Error: %e\n", amax);
+  printf("RMS Error: %Le\n", sqrtl(sq / s.cells));
+  const double rtol = kDtype == AN5D_FLOAT64 ? 1e-12 : 1e-5;
+  printf("Max Rel Error: %e (floor %e, non-finite mismatches %lld)\n", rmax, floor_rel, bad);
+  const bool ok = rmax <= rtol && bad == 0;
+  printf("Verdict: %s (rtol %g)\n", ok ? "PASS" : "FAIL", rtol);
+  return ok;
 }
 
-// float runs: GPU result against the CPU path in FP64 on the same input
-static void check_fp64(const float* A, int compsize, int timestep) {
-  size_t gridarea = 1, dimsize = compsize + g_info.rad * 2;
-  for (int i = 0; i < g_info.dim; i++) gridarea *= dimsize;
-  double* S = (double*)malloc(2 * gridarea * sizeof(double));
-  float* F = (float*)malloc(2 * gridarea * sizeof(float));
-  if (!S || !F) {
+// Float runs: the GPU result against the CPU path in FP64 from the same input.
+void compare_fp64(const SB_TYPE* gpu, const Shape& s, int n, int T) {
+  if (kDtype != AN5D_FLOAT32) return;
+  std::vector<double> D;
+  try {
+    D.resize((size_t)2 * s.cells);
+  } catch (...) {
     printf("Max Error vs FP64: skipped (host memory)\n");
-    free(S);
-    free(F);
     return;
   }
-  init_grid<float>(F, compsize);
-  for (size_t i = 0; i < 2 * gridarea; i++) S[i] = F[i];
+  std::vector<float> F((size_t)2 * s.cells);
+  fill(reinterpret_cast<SB_TYPE*>(F.data()), s, n);
+  for (size_t i = 0; i < D.size(); i++) D[i] = F[i];
   an5d_problem p;
   memset(&p, 0, sizeof(p));
   p.stencil = an5d_stencil_id(AN5D_STENCIL);
   p.dtype = AN5D_FLOAT64;
-  for (int d = 0; d < g_info.dim; d++) p.n[d] = compsize;
-  p.timesteps = timestep;
+  for (int d = 0; d < s.info.dim; d++) p.n[d] = n;
+  p.timesteps = T;
   p.divisor = -1.0;
   double ms = 0.0;
-  if (an5d_cpu_forward(&p, S, 0, &ms) != AN5D_OK) {
+  if (an5d_cpu_forward(&p, D.data(), 0, &ms) != AN5D_OK) {
     printf("Max Error vs FP64: skipped (%s)\n", an5d_last_error());
-  } else {
-    int t = timestep % 2;
-    double emax = 0.0;
-    for (size_t i = 0; i < gridarea; i++) {
-      double e = fabs((double)A[t * gridarea + i] - S[t * gridarea + i]);
-      emax = e > emax ? e : emax;
-    }
-    printf("Max Error vs FP64: %e\n", emax);
+    return;
   }
-  free(S);
-  free(F);
+  const SB_TYPE* g = gpu + (size_t)(T % 2) * s.cells;
+  const double* r = D.data() + (size_t)(T % 2) * s.cells;
+  double e = 0.0;
+  for (long long i = 0; i < s.cells; i++) e = fmax(e, fabs((double)g[i] - r[i]));
+  printf("Max Error vs FP64: %e\n", e);
 }
 
-static double calc_gflops(double total_flop, double ms) { return (total_flop / ms) / (1000 * 1000); }
-
-static int benchmark(int compsize, int timestep, int reptnum, bool cpucomp) {
-  size_t gridarea = 1, comparea = 1;
-  for (int i = 0; i < g_info.dim; i++) {
-    gridarea *= (compsize + g_info.rad * 2);
-    comparea *= compsize;
-  }
-  double total_flop = (double)timestep * comparea * g_info.fpp;
-  SB_TYPE* A = (SB_TYPE*)malloc(2 * gridarea * sizeof(SB_TYPE));
-  SB_TYPE* B = (SB_TYPE*)malloc(2 * gridarea * sizeof(SB_TYPE));
-  if (A == NULL || B == NULL) {
+// The reference's run sequence (common.h:85-129): one warm-up call, `reps`
+// timed calls each on a freshly initialised grid, their average, then the
+// optional CPU comparison.
+int run(const Shape& s, int n, int T, int reps, bool check) {
+  std::vector<SB_TYPE> gpu, cpu;
+  try {
+    gpu.resize((size_t)2 * s.cells);
+    if (check) cpu.resize((size_t)2 * s.cells);
+  } catch (...) {
     fprintf(stderr, "Host memory allocation failed\n");
     exit(EXIT_FAILURE);
   }
+  const double flops = (double)T * (double)s.interior * s.info.fpp;
+  auto gflops = [&](double ms) { return flops / ms / 1e6; };
   puts("[Device warm-up]");
-  printf("%.4f ms\n", kernel_stencil(A, compsize, timestep, true));
-  double elapsed = 0.0f;
+  printf("%.4f ms\n", kernel_stencil(gpu.data(), n, T, true));
   puts("\n[Device execution]");
-  for (int n = 0; n < reptnum; n++) {
-    init_grid<SB_TYPE>(A, compsize);
-    double ms = kernel_stencil(A, compsize, timestep, true);
-    printf("%7d: %10.4lf GFLOPS, %10.4f ms\n", n, calc_gflops(total_flop, ms), ms);
-    elapsed += ms;
+  double total_ms = 0.0;
+  for (int k = 0; k < reps; k++) {
+    fill(gpu.data(), s, n);
+    const double ms = kernel_stencil(gpu.data(), n, T, true);
+    printf("%7d: %10.4lf GFLOPS, %10.4f ms\n", k, gflops(ms), ms);
+    total_ms += ms;
   }
-  printf("Average: %10.4lf GFLOPS, %10.4f ms\n", calc_gflops(total_flop, elapsed / reptnum),
-         elapsed / reptnum);
-  printf("GStencil/s: %10.4lf\n", (double)timestep * comparea / (elapsed / reptnum) / 1e6);
-  int rc = EXIT_SUCCESS;
-  if (cpucomp) {
-    puts("\n[CPU execution]");
-    init_grid<SB_TYPE>(B, compsize);
-    printf("%.4f ms\n", kernel_stencil(B, compsize, timestep, false));
-    if (!check_result<SB_TYPE>(A, B, compsize, timestep)) rc = EXIT_FAILURE;
-    if (sizeof(SB_TYPE) == 4) check_fp64((const float*)A, compsize, timestep);
-  }
-  free(A);
-  free(B);
-  return rc;
+  const double avg = total_ms / reps;
+  printf("Average: %10.4lf GFLOPS, %10.4f ms\n", gflops(avg), avg);
+  printf("GStencil/s: %10.4lf\n", (double)T * s.interior / avg / 1e6);
+  if (!check) return EXIT_SUCCESS;
+  puts("\n[CPU execution]");
+  fill(cpu.data(), s, n);
+  printf("%.4f ms\n", kernel_stencil(cpu.data(), n, T, false));
+  const bool ok = compare(gpu.data(), cpu.data(), s, T);
+  compare_fp64(gpu.data(), s, n, T);
+  return ok ? EXIT_SUCCESS : EXIT_FAILURE;
 }
 
+// -s/-t/-n values below 1 end the program with the reference's messages
+// (common.h:147-166).
+int positive_arg(const char* text, const char* what) {
+  const int v = (int)strtol(text, nullptr, 10);
+  if (v < 1) {
+    fprintf(stderr, "%s: %d\n", what, v);
+    exit(EXIT_FAILURE);
+  }
+  return v;
+}
+
+}  // namespace
+
 int main(int argc, char** argv) {
-  int id = an5d_stencil_id(AN5D_STENCIL);
-  if (id < 0 || an5d_stencil_get_info(id, &g_info) != AN5D_OK) {
+  an5d_stencil_info info;
+  const int id = an5d_stencil_id(AN5D_STENCIL);
+  if (id < 0 || an5d_stencil_get_info(id, &info) != AN5D_OK) {
     fprintf(stderr, "[AN5D ERROR] %s\n", an5d_last_error());
     return EXIT_FAILURE;
   }
-  int compsize = (g_info.dim == 2) ? 8192 : 512;
-  int timestep = 120;
-  int reptnum = 2;
-  bool cpucomp = false;
-  int opt;
-  while ((opt = getopt(argc, argv, "s:t:n:ch")) != -1) {
-    switch (opt) {
-      case 's':
-        compsize = (int)strtol(optarg, NULL, 10);
-        if (compsize < 1) {
-          fprintf(stderr, "Larger compute size required: %d\n", compsize);
-          exit(EXIT_FAILURE);
-        }
-        break;
-      case 't':
-        timestep = (int)strtol(optarg, NULL, 10);
-        if (timestep < 1) {
-          fprintf(stderr, "More than one time-step required: %d\n", timestep);
-          exit(EXIT_FAILURE);
-        }
-        break;
-      case 'n':
-        reptnum = (int)strtol(optarg, NULL, 10);
-        if (reptnum < 1) {
-          fprintf(stderr, "More than one execution required: %d\n", reptnum);
-          exit(EXIT_FAILURE);
-        }
-        break;
-      case 'c':
-        cpucomp = true;
-        break;
-      case 'h':
-        usage(argv[0]);
-        return EXIT_SUCCESS;
+  int n = info.dim == 2 ? 8192 : 512, T = 120, reps = 2;
+  bool check = false;
+  for (int opt; (opt = getopt(argc, argv, "s:t:n:ch")) != -1;) {
+    if (opt == 's') n = positive_arg(optarg, "Larger compute size required");
+    else if (opt == 't') T = positive_arg(optarg, "More than one time-step required");
+    else if (opt == 'n') reps = positive_arg(optarg, "More than one execution required");
+    else if (opt == 'c') check = true;
+    else if (opt == 'h') {
+      print_usage(argv[0]);
+      return EXIT_SUCCESS;
     }
   }
-  return benchmark(compsize, timestep, reptnum, cpucomp);
+  return run(shape_of(info, n), n, T, reps, check);
 }

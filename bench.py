@@ -21,6 +21,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -54,26 +55,78 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """CPU model, the hardware threads this process may use and the physical
+    cores they belong to (from /proc/cpuinfo and the affinity mask)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    model, cores, cur = "unknown", set(), {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if cur.get("processor") in allowed:
+                    cores.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+                cur = {}
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            cur[k] = int(v) if k == "processor" else v
+            if k == "model name":
+                model = v
+        if cur.get("processor") in allowed:
+            cores.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+    except OSError:
+        pass
+    return model, len(allowed), len(cores) or len(allowed)
+
+
 def cpu_baseline(wl, budget_s=12.0):
-    """The reference's OpenMP CPU loop (oracle/_ref) timed on a bounded sample:
-    the same grid, fewer timesteps.  Falls back to the oracle's sequential loop
-    on a smaller grid when the reference build is absent."""
+    """The reference's own CPU paths (oracle/_ref, built from <stencil>.c with
+    -O3 -fopenmp -ffp-contract=off): the OpenMP loop on every core this
+    process may use (one thread per core, OMP_PROC_BIND=close,
+    OMP_PLACES=cores) and the sequential scop loop on one, each timed on a
+    bounded sample of the same grid (GStencil/s is a per-step rate).  It runs
+    in a child process, so that its OpenMP placement is its own and the GPU
+    process keeps its threads unbound."""
+    model, hw_threads, phys = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", phys))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    code = ("import json, sys; sys.path.insert(0, sys.argv[1]); import bench; "
+            "print(json.dumps(bench.cpu_baseline_child(sys.argv[2], float(sys.argv[3]))))")
+    key = next(k for k, v in WORKLOADS.items() if v is wl)
+    r = subprocess.run([sys.executable, "-c", code, ROOT, key, str(budget_s)], env=env, capture_output=True,
+                       text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {r.stderr[-2000:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out.update({"cores": threads, "cpu_model": model, "physical_cores": phys, "hw_threads": hw_threads,
+                "omp": {k: env[k] for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}})
+    return out
+
+
+def cpu_baseline_child(key, budget_s):
+    """Body of the CPU baseline (runs in its own process, see cpu_baseline).
+    Falls back to the oracle's sequential loop on a smaller grid when the
+    reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    wl = WORKLOADS[key]
     name, dt = wl["stencil"], wl["dtype"]
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cells = float(np.prod(wl["n"]))
     if O.ref_available(name, dt):
         n = wl["n"][0]
         A = O.ref_init_grid(name, n, dt)
         t1 = O.ref_run(name, A, n, 1, scop=False) / 1e3
-        steps = int(max(1, min(200, budget_s / max(t1, 1e-6))))
+        steps = int(max(1, min(wl["T"], budget_s / max(t1, 1e-6))))
         ms = O.ref_run(name, A, n, steps, scop=False)
-        cells = float(np.prod(wl["n"]))
-        return {"value": steps * cells / (ms / 1e3) / 1e9, "unit": "GStencil/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"{name}.c OpenMP loop (scop=false), {'x'.join(map(str, wl['n']))}, {steps} steps, "
-                          f"OMP_NUM_THREADS={threads}"}
-    d = O.info(name)
+        # the sequential scop loop: a fifth of the time budget
+        s1 = O.ref_run(name, A, n, 1, scop=True) / 1e3
+        steps1 = int(max(1, min(wl["T"], 0.2 * budget_s / max(s1, 1e-6))))
+        ms1 = O.ref_run(name, A, n, steps1, scop=True)
+        return {"value": steps * cells / (ms / 1e3) / 1e9, "unit": "GStencil/s", "kind": "reference",
+                "sample": f"{name}.c OpenMP loop (kernel_stencil scop=false), {'x'.join(map(str, wl['n']))}, "
+                          f"{steps} of {wl['T']} steps, reference coefficients and init_grid",
+                "build": "gcc -O3 -fopenmp -ffp-contract=off",
+                "value_1thread": steps1 * cells / (ms1 / 1e3) / 1e9,
+                "sample_1thread": f"{name}.c sequential scop loop, {steps1} steps"}
     n = [min(x, 128) for x in wl["n"]]
     A = O.init_grid(name, n[0], dt) if len(set(n)) == 1 else None
     t0 = time.perf_counter()
@@ -83,8 +136,44 @@ def cpu_baseline(wl, budget_s=12.0):
     t0 = time.perf_counter()
     O.run(name, A, n, steps)
     dt_s = time.perf_counter() - t0
-    return {"value": steps * float(np.prod(n)) / dt_s / 1e9, "unit": "GStencil/s", "cores": 1,
-            "kind": "port", "sample": f"oracle sequential loop, {'x'.join(map(str, n))}, {steps} steps"}
+    return {"value": steps * float(np.prod(n)) / dt_s / 1e9, "unit": "GStencil/s", "kind": "port",
+            "sample": f"oracle sequential loop, {'x'.join(map(str, n))}, {steps} steps"}
+
+
+def error_statement(torch, pyan5d, wl, coefs, divisor, variant, sl, mode):
+    """Per-run correctness statement: the first 2*bt steps of the benchmarked
+    configuration (same kernel, tiling, coefficients and input) against the
+    oracle's C loop in the same precision (max abs, max rel with the smallest
+    normal as floor, PASS at 1e-5 FP32 / 1e-12 FP64) and in FP64."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    name, dt, n = wl["stencil"], wl["dtype"], wl["n"]
+    rc = pyan5d.resolve_config(name, n, dt, variant=variant, sl=sl, mode=mode)
+    T = 2 * rc["bt"]
+    A = pyan5d.init_grid(name, n, dt, mode=1)
+    ref = A.copy()
+    ref64 = A.astype(np.float64)
+    g = torch.from_numpy(A).cuda()
+    pyan5d.forward_device(name, g, n, T, coefs=coefs, divisor=divisor, bt=rc["bt"], variant=variant, sl=sl,
+                          mode=mode)
+    torch.cuda.synchronize()
+    got = g[T % 2].cpu().numpy().astype(np.float64)
+    del g
+    O.run(name, ref, n, T, coefs=coefs, divisor=divisor)
+    O.run(name, ref64, n, T, coefs=coefs, divisor=divisor)
+    c = ref[T % 2].astype(np.float64)
+    c64 = ref64[T % 2]
+    floor = float(np.finfo(dt).tiny)
+    max_abs = float(np.max(np.abs(got - c)))
+    max_rel = float(np.max(np.abs(got - c) / np.maximum(np.abs(c), floor)))
+    rms = float(np.sqrt(np.mean((got - c) ** 2)))
+    err64 = float(np.max(np.abs(got - c64)))
+    rel64 = float(np.max(np.abs(got - c64) / np.maximum(np.abs(c64), floor)))
+    tol = 1e-5 if dt == np.float32 else 1e-12
+    finite = bool(np.all(np.isfinite(got)) == np.all(np.isfinite(c)))
+    return {"steps_checked": T, "max_abs": max_abs, "rms": rms, "max_rel": max_rel, "rel_floor": floor,
+            "max_err_vs_fp64": err64, "max_rel_vs_fp64": rel64, "tol_rel": tol,
+            "bitwise": bool(max_abs == 0.0 and finite), "pass": bool(max_rel <= tol and finite)}
 
 
 def bench_fwd_adj(torch, pyan5d, presets, name, n=(8192, 8192), T=256, reps=3, abts=(2, 3, 4, 5), sls=(0, 64)):
@@ -329,7 +418,8 @@ def main():
     gstencil = T * cells / (ms_per_step / 1e3) / 1e9
     alg_bytes = 2.0 * esz * cells  # per launch of bt steps: one read + one write of the grid
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
-    tag = f"{name}_{'f32' if dt == np.float32 else 'f64'}_bt{bt}"
+    # the measured bytes belong to one kernel: variant, depth and segment length
+    tag = f"{name}_{'f32' if dt == np.float32 else 'f64'}_v{variant}_bt{bt}_sl{sl}"
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": parse_profile_traffic(tag)}
 
@@ -345,6 +435,10 @@ def main():
         extra["batched"] = bench_batched(torch, pyan5d, presets)
         log(f"[bench] {extra['batched']}")
         torch.cuda.empty_cache()
+
+    log("[bench] error statement (first launches vs the oracle) ...")
+    errs = error_statement(torch, pyan5d, wl, coefs, divisor, variant, sl, args.mode)
+    log(f"[bench] error statement {errs}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -366,6 +460,7 @@ def main():
                    "frac_of_bt_scaled_roof": round(gstencil / (HBM_PEAK_GBS * bt / (2 * esz)), 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "error": errs,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

@@ -164,7 +164,9 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
   const long long n0 = g.D0 - 2 * R, nj = g.D1 - (g.dim == 3 ? 2 * R : 0), nk = g.D2 - 2 * R;
   LaunchArgs& a = lp.args;
   a.D0 = (int)g.D0; a.D1 = (int)g.D1; a.D2 = (int)g.D2;
-  lp.fn = (bt >= 1 && bt <= v.btmax) ? (mode == AN5D_FAST ? v.fn_fast[bt] : v.fn[bt]) : nullptr;
+  lp.fn = nullptr;
+  if (bt >= 1 && bt <= v.btmax)
+    lp.fn = mode == AN5D_FAST ? v.fn_fast[bt] : (mode == kLaunchShare ? v.fn_share[bt] : v.fn[bt]);
   if (!lp.fn) { err = "no kernel for bt=" + std::to_string(bt) + " in this variant"; return false; }
   lp.lds = (bt >= 1 && bt <= kMaxBT) ? v.lds[bt] : 0;
   if (v.kind == kDirect) {
@@ -264,6 +266,27 @@ bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& 
     float rc = has_div ? 1.0f / (float)vals[nt] : 0.0f;
     memcpy(&bytes[(nt + 1) * 4], &rc, 4);
   }
+  return true;
+}
+
+// The shared-product form applies when the stencil is a radius-1 star (a
+// centre and its 2*dim face taps) and the coefficient table gives every face
+// tap the same value in the working precision (bit for bit: each product
+// c * u is then one rounded value, whichever sum it enters).
+bool coefs_shareable(const an5d_problem* p, const std::vector<unsigned char>& bytes) {
+  const StencilRT& s = stencil_rt(p->stencil);
+  if (!s.linear || s.rad != 1 || s.ntaps != 2 * s.dim + 1) return false;
+  int centre = -1;
+  for (int q = 0; q < s.ntaps; q++) {
+    const TapDef& t = s.taps[q];
+    if (t.d1 != 0 && t.d2 != 0) return false;
+    if (t.d0 == 0 && t.d1 == 0 && t.d2 == 0) centre = q;
+  }
+  if (centre < 0) return false;
+  const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
+  const int ref = centre == 0 ? 1 : 0;
+  for (int q = 0; q < s.ntaps; q++)
+    if (q != centre && memcmp(&bytes[q * esz], &bytes[ref * esz], esz) != 0) return false;
   return true;
 }
 
@@ -372,11 +395,13 @@ extern "C" int an5d_resolve_config(const an5d_problem* p, const an5d_config* cfg
   if (!make_geom(p, g, err)) return fail(AN5D_E_INVALID, err);
   const Variant* v = nullptr;
   int bt = 0, sl = 0;
-  int rc = resolve(p, cfg, &v, &bt, &sl);
-  if (rc) return rc;
   const int mode = cfg ? cfg->mode : AN5D_EXACT;
+  an5d_config rcfg = cfg ? *cfg : an5d_config{0, -1, 0, AN5D_EXACT};
+  if (mode == AN5D_EXACT_GENERAL) rcfg.mode = AN5D_EXACT;  // tiled like exact mode
+  int rc = resolve(p, &rcfg, &v, &bt, &sl);
+  if (rc) return rc;
   LaunchPlan lp;
-  if (!plan_launch(*v, bt, g, sl, lp, err, mode)) return fail(AN5D_E_UNSUPPORTED, err);
+  if (!plan_launch(*v, bt, g, sl, lp, err, rcfg.mode)) return fail(AN5D_E_UNSUPPORTED, err);
   auto vs = variants_of(p->stencil, p->dtype);
   int vi = 0;
   while (vi < (int)vs.size() && vs[vi] != v) vi++;
@@ -480,16 +505,27 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
   if (L.s1 < g.D2 || (g.dim == 3 && L.s0 < g.D1 * L.s1) || (g.dim == 2 && L.s0 < g.D2) ||
       L.buf_stride < g.D0 * L.s0)
     return fail(AN5D_E_INVALID, "layout strides smaller than the grid");
+  const int mode = cfg ? cfg->mode : AN5D_EXACT;
+  if (mode != AN5D_EXACT && mode != AN5D_FAST && mode != AN5D_EXACT_GENERAL)
+    return fail(AN5D_E_INVALID, "bad mode");
+  // AN5D_EXACT_GENERAL tiles like AN5D_EXACT (same table entries)
+  an5d_config rcfg = cfg ? *cfg : an5d_config{0, -1, 0, AN5D_EXACT};
+  if (mode == AN5D_EXACT_GENERAL) rcfg.mode = AN5D_EXACT;
   const Variant* v = nullptr;
   int bt = 0, sl = 0;
-  int rc = resolve(p, cfg, &v, &bt, &sl);
+  int rc = resolve(p, &rcfg, &v, &bt, &sl);
   if (rc) return rc;
   std::vector<unsigned char> cbytes;
   int has_div = 0;
   double gscale = 1.0;
-  const int mode = cfg ? cfg->mode : AN5D_EXACT;
-  if (mode != AN5D_EXACT && mode != AN5D_FAST) return fail(AN5D_E_INVALID, "bad mode");
-  if (!build_coefs(p, cbytes, has_div, gscale, err, mode)) return fail(AN5D_E_INVALID, err);
+  if (!build_coefs(p, cbytes, has_div, gscale, err, rcfg.mode)) return fail(AN5D_E_INVALID, err);
+  // launch mode per step count: the shared-product form where it applies
+  // (bitwise the same) and its kernel does not spill
+  const bool share = mode == AN5D_EXACT && v->kind == kSweep && coefs_shareable(p, cbytes);
+  auto launch_mode = [&](int k) {
+    if (share && v->fn_share[k] && v->share_scratch[k] && v->share_scratch[k]() == 0) return kLaunchShare;
+    return rcfg.mode;
+  };
   const void* dcoef = nullptr;
   rc = device_coefs(cbytes, &dcoef);
   if (rc) return rc;
@@ -510,7 +546,7 @@ int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
   for (int k : steps) {
     if (k < 1 || k > kMaxBT) return fail(AN5D_E_INVALID, "bad step count");
     if (!planned[k]) {
-      if (!plan_launch(*v, k, g, sl, plans[k], err, mode)) return fail(AN5D_E_UNSUPPORTED, err);
+      if (!plan_launch(*v, k, g, sl, plans[k], err, launch_mode(k))) return fail(AN5D_E_UNSUPPORTED, err);
       planned[k] = true;
     }
     LaunchPlan lp = plans[k];

@@ -60,7 +60,14 @@ struct Variant {
   size_t lds[kMaxBT + 1];  // dynamic LDS bytes per workgroup, by bt
   LaunchFn fn[kMaxBT + 1];       // fn[bt], bt in [1, btmax], exact mode
   LaunchFn fn_fast[kMaxBT + 1];  // fast (FMA) mode, same shapes
-  int (*occ[kMaxBT + 1])(int mode);  // resident workgroups per CU of fn[bt] / fn_fast[bt]
+  // exact mode in the shared-product form (radius-1 stars whose face taps
+  // carry one coefficient); null where the stencil has no such form
+  LaunchFn fn_share[kMaxBT + 1];
+  // private (scratch) bytes per thread of fn_share[bt] (-1 unknown): a kernel
+  // that spills is slower than the general form and is not used
+  int (*share_scratch[kMaxBT + 1])();
+  // resident workgroups per CU of fn[bt] (mode 0), fn_fast[bt] (1), fn_share[bt] (2)
+  int (*occ[kMaxBT + 1])(int mode);
   LaunchFn fn_mid[kMaxBT + 1];  // exact mode storing intermediate stages (2D, bt <= kMaxMidBT)
 };
 

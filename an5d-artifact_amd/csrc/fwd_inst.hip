@@ -35,26 +35,44 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
   return s;
 }
 
-template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS = false>
+constexpr int kUnroll = default_unroll<Def<SID>::rad>();
+constexpr bool kShare = share_capable<Def<SID>>();
+
+template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS = false, bool SHARE = false>
 hipError_t launch_sweep(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
   if (lds > 65536) {
     // gfx950 has 160 KB of LDS per CU
     static std::atomic<int> done[kMaxDevices];
     const hipError_t e = lds_opt_in(
-        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>), lds, done);
+        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS, kUnroll, SHARE>), lds,
+        done);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS>), grid, block, lds, st,
+  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS, kUnroll, SHARE>), grid, block, lds, st,
                      to_args<T>(a));
   return hipGetLastError();
 }
 
+// Private bytes per thread of the shared-product kernel (its register
+// spills), asked once; -1 if the query fails.
+template <int BT, int CK, int CJ, int NW>
+int share_scratch_once() {
+  static const int n = [] {
+    hipFuncAttributes attr;
+    const void* k = reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, true, false, kUnroll, true>);
+    if (hipFuncGetAttributes(&attr, k) != hipSuccess) return -1;
+    return (int)attr.localSizeBytes;
+  }();
+  return n;
+}
+
 // Resident workgroups per CU (0 if the query fails); computed once, on the
 // first device that asks (every device is a gfx950 with the same resources).
-template <int BT, int CK, int CJ, int NW, bool EXACT>
+template <int BT, int CK, int CJ, int NW, bool EXACT, bool SHARE = false>
 int occupancy_once() {
   static const int n = [] {
-    const void* k = reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT>);
+    const void* k =
+        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, false, kUnroll, SHARE>);
     const size_t lds = sweep_lds_bytes<SID, T, CK, CJ, NW>(BT);
     static std::atomic<int> done[kMaxDevices];
     if (lds > 65536 && lds_opt_in(k, lds, done) != hipSuccess) return 0;
@@ -66,7 +84,9 @@ int occupancy_once() {
 }
 template <int BT, int CK, int CJ, int NW>
 int occupancy_sweep(int mode) {
-  return mode ? occupancy_once<BT, CK, CJ, NW, false>() : occupancy_once<BT, CK, CJ, NW, true>();
+  if constexpr (kShare)
+    if (mode == 2) return occupancy_once<BT, CK, CJ, NW, true, true>();
+  return mode == 1 ? occupancy_once<BT, CK, CJ, NW, false>() : occupancy_once<BT, CK, CJ, NW, true>();
 }
 
 template <int CK>
@@ -131,6 +151,10 @@ void add_sweep(std::vector<Variant>& out) {
       v.lds[BT] = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>(BT);
       v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true>;
       v.fn_fast[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, false>;
+      if constexpr (kShare) {
+        v.fn_share[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true, false, true>;
+        v.share_scratch[BT] = &share_scratch_once<BT, sh.ck, sh.cj, sh.nw>;
+      }
       v.occ[BT] = &occupancy_sweep<BT, sh.ck, sh.cj, sh.nw>;
       // intermediate-stage outputs: the adjoint's checkpointing forward pass (2D, exact)
       if constexpr (Def<SID>::dim == 2 && BT >= 2 && BT <= kMaxMidBT)
@@ -149,6 +173,7 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   add_sweep<1>(out);
   add_sweep<2>(out);
   add_sweep<3>(out);
+  add_sweep<4>(out);
   Variant d{};
   d.stencil = SID;
   d.dtype = AN5D_F64 ? kF64 : kF32;

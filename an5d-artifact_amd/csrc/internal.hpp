@@ -42,8 +42,12 @@ an5d_layout reference_layout(const Geom& g);
 an5d_layout padded_layout(const Geom& g);
 int resolve(const an5d_problem* p, const an5d_config* cfg, const Variant** vout, int* btout,
             int* slout);
+// plan_launch's mode: AN5D_EXACT, AN5D_FAST, or kLaunchShare (exact, the
+// shared-product kernels)
+constexpr int kLaunchShare = 2;
 bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp,
                  std::string& err, int mode = 0);
+bool coefs_shareable(const an5d_problem* p, const std::vector<unsigned char>& bytes);
 bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,
                  double& gscale, std::string& err, int mode = 0);
 int device_coefs(const std::vector<unsigned char>& bytes, const void** out);

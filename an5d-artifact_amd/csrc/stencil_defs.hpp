@@ -91,6 +91,27 @@ constexpr int ext_slot(int d0) {
   return -1;
 }
 
+// index of the centre tap (offset 0, 0, 0), -1 if there is none
+template <class D>
+constexpr int center_tap() {
+  for (int q = 0; q < D::ntaps; q++)
+    if (D::taps[q].d0 == 0 && D::taps[q].d1 == 0 && D::taps[q].d2 == 0) return q;
+  return -1;
+}
+// Radius-1 stars (a centre and its 2*dim face neighbours): when the runtime
+// coefficients of all face taps are equal, each cell's product c * u is the
+// same rounded value in all 2*dim sums that use it, so it can be formed once
+// per cell (the shared-product form of the sweep).
+template <class D>
+constexpr bool share_capable() {
+  return D::linear && D::rad == 1 && !needs_corners<D>() && center_tap<D>() >= 0 && D::ntaps == 2 * D::dim + 1;
+}
+// the tap whose coefficient the face taps share
+template <class D>
+constexpr int share_tap() {
+  return center_tap<D>() == 0 ? 1 : 0;
+}
+
 // Compile-time loop over [B, E): f(std::integral_constant<int, i>) in order.
 template <int B, class F, int... Is>
 __host__ __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {

@@ -240,6 +240,36 @@ __device__ __forceinline__ T eval_point(const Get& get, const T* c, T div, int h
   }
 }
 
+// Shared-product form (radius-1 stars whose face taps all carry the same
+// coefficient c_f): `get` returns the products P = c_f * u of the face
+// neighbours (the in-plane ones formed once per cell of the centre plane and
+// moved by DPP / the LDS exchange, the two off-plane ones formed here); the
+// centre term is c_0 * u as in the C loop.  Every term is the same rounded
+// product as in eval_point, summed in the same order: the result is bit for
+// bit the same.
+template <class D, typename T, int DM = -1, class Get>
+__device__ __forceinline__ T eval_share(const Get& get, T ucen, const T* c, T div, int has_div) {
+#pragma clang fp contract(off)
+  static_assert(share_capable<D>(), "shared products need a radius-1 star");
+  T acc;
+  static_for<0, D::ntaps>([&](auto qc) {
+    constexpr int Q = decltype(qc)::value;
+    constexpr TapDef t = D::taps[Q];
+    T v;
+    if constexpr (Q == center_tap<D>()) v = c[Q] * ucen;
+    else v = get.template at<t.d0, t.d1, t.d2>();
+    if constexpr (Q == 0) acc = v;
+    else if constexpr (t.sign > 0) acc = acc + v;
+    else acc = acc - v;
+  });
+  if constexpr (DM < 0) {
+    if (has_div) acc = div_by(acc, div, c[D::ntaps + 1], has_div);
+  } else if constexpr (DM > 0) {
+    acc = div_by(acc, div, c[D::ntaps + 1], DM);
+  }
+  return acc;
+}
+
 template <typename T, int CJ, int CK>
 struct Plane { T v[CJ][CK]; };
 
@@ -254,6 +284,20 @@ struct RegGet {
     constexpr int SL = ext_slot<D>(A0);
     if constexpr (SL >= 0) return x[SL][JJ + A1 + XJ][KK + A2 + XK];
     else return rin[D::rad + A0].v[JJ][KK];
+  }
+};
+
+// Shared-product form: in-plane face products from the extension x of the
+// centre plane's P, off-plane ones formed from the ring.
+template <class D, typename T, int CJ, int CK, int XJ, int XK, int JJ, int KK>
+struct ShareGet {
+  const T (&x)[1][CJ + 2 * XJ][CK + 2 * XK];
+  const Plane<T, CJ, CK> (&rin)[2 * D::rad + 1];
+  T cf;
+  template <int A0, int A1, int A2>
+  __device__ __forceinline__ T at() const {
+    if constexpr (A0 == 0) return x[0][JJ + A1 + XJ][KK + A2 + XK];
+    else return cf * rin[D::rad + A0].v[JJ][KK];
   }
 };
 
@@ -342,8 +386,12 @@ constexpr size_t sweep_lds_bytes(int bt) {
 // blockIdx (used by the batched work-queue kernel).
 // MIDS: the kernel also stores intermediate stages (SweepArgs::mid_mask);
 // compiled only where used, since the extra store paths enlarge every stage.
+// SHARE: the shared-product form (eval_share): each stage forms the products
+// P = c_f * u of its centre plane once, and the in-plane neighbours (DPP and
+// the middle-dim exchange, whose edge rows are published as P) move P
+// instead of u.
 template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT, int DM, int UNR,
-          bool MIDS = false>
+          bool MIDS = false, bool SHARE = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1) {
   using D = Def<ID>;
   constexpr int R = D::rad;
@@ -364,6 +412,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   static_assert(!FAST || CK >= R, "a lane must hold at least R cells");
   static_assert(DIM3 || CJ == 1, "2D sweeps hold one row per lane");
   static_assert(!MID || NW * CJ > 2 * H, "tile too thin for BT");
+  static_assert(!SHARE || (share_capable<D>() && EXACT), "shared products: exact radius-1 stars only");
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -430,6 +479,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   int xbuf = 0;
 
   Plane<T, CJ, CK> ring[BT][P];
+  // shared-product form: the face coefficient all face taps carry
+  const T cf = c[share_capable<D>() ? share_tap<D>() : 0];
+  auto scaled = [&](const Plane<T, CJ, CK>& in) {
+    Plane<T, CJ, CK> o;
+#pragma unroll
+    for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) o.v[jj][kk] = cf * in.v[jj][kk];
+    return o;
+  };
 
   // Every load reads real grid cells: plane, row and column indices are
   // clamped into the grid (the stand-in values only reach cells outside it,
@@ -476,18 +535,26 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 
   // Stage S output plane from ring[S-1]; q = global index of that plane.
   constexpr bool STARX = star_exchange<D>();
+  constexpr bool STARX_DB = star_exchange_db<D>(BT);
   constexpr int ERS = edge_rows<CJ, XJ>();
-  T xh[STARX ? BT : 1][STARX ? 2 * XJ : 1][CK];  // halo rows of each stage's centre plane
+  constexpr size_t XBUF = (size_t)BT * NW * ERS * 64 * CK;  // elements per exchange buffer
+  // single-buffered exchange: the halo rows of each stage's centre plane, read
+  // once per iteration; double-buffered: read by each stage when it needs them
+  T xh[STARX && !STARX_DB ? BT : 1][STARX && !STARX_DB ? 2 * XJ : 1][CK];
+  int xrd = 0;  // double-buffered exchange: the buffer this iteration reads
 
   auto compute = [&](const Plane<T, CJ, CK>(&rin)[P], Plane<T, CJ, CK>& out, int q, auto sc) {
     constexpr int S = decltype(sc)::value;
-    // extended copies of the planes that need in-plane neighbours
+    // extended copies of the planes that need in-plane neighbours (in the
+    // shared-product form: of the centre plane's products P = c_f * u)
     T x[NX > 0 ? NX : 1][CJ + 2 * XJ][CK + 2 * XK];
+    Plane<T, CJ, CK> pcen;
+    if constexpr (SHARE) pcen = scaled(rin[R]);
     static_for<-R, R + 1>([&](auto dc) {
       constexpr int DD = decltype(dc)::value;
       constexpr int SL = ext_slot<D>(DD);
       if constexpr (SL >= 0) {
-        const auto& src = rin[R + DD];
+        const auto& src = SHARE ? pcen : rin[R + DD];
 #pragma unroll
         for (int jj = 0; jj < CJ; jj++) {
 #pragma unroll
@@ -502,12 +569,30 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         }
       }
     });
-    if constexpr (STARX) {
+    if constexpr (STARX_DB) {
+      // the neighbours' edge rows of this stage's centre plane, written during
+      // the previous iteration into buffer xrd
+      const T* lds = reinterpret_cast<const T*>(an5d_lds) + (size_t)xrd * XBUF;
+      static_for<0, 2 * XJ>([&](auto hc) {
+        constexpr int HI = decltype(hc)::value;
+        constexpr int JR = HI < XJ ? HI - XJ : CJ + (HI - XJ);
+        constexpr int WOFF = JR >= 0 ? JR / CJ : -((-JR + CJ - 1) / CJ);
+        constexpr int RR = JR - WOFF * CJ;
+        constexpr int E = edge_index<CJ, XJ>(RR);
+        static_assert(E >= 0, "halo row must be published");
+        int ws = w + WOFF;
+        ws = ws < 0 ? 0 : (ws >= NW ? NW - 1 : ws);
+        T run[CK];
+        load_run<T, CK>(run, lds + ((((size_t)(S - 1) * NW + ws) * ERS + E) * 64 + lane) * CK);
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) x[0][JR + XJ][kk + XK] = run[kk];
+      });
+    } else if constexpr (STARX) {
 #pragma unroll
       for (int h = 0; h < 2 * XJ; h++) {
         const int jr = h < XJ ? h : CJ + XJ + (h - XJ);  // x row index of the halo row
 #pragma unroll
-        for (int kk = 0; kk < CK; kk++) x[0][jr][kk + XK] = xh[S - 1][h][kk];
+        for (int kk = 0; kk < CK; kk++) x[0][jr][kk + XK] = xh[(S - 1) * !STARX_DB][h * !STARX_DB][kk];
       }
     } else if constexpr (MID) {
       // publish own edge rows, then fetch the XJ rows above and below
@@ -562,10 +647,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       constexpr int JJ = decltype(jc)::value;
       static_for<0, CK>([&](auto kc) {
         constexpr int KK = decltype(kc)::value;
-        RegGet<D, T, (NX > 0 ? NX : 1), CJ, CK, XJ, XK, JJ, KK> get{x, rin};
-        if constexpr (BATCHDIV) val[JJ][KK] = eval_point<D, T, 0>(get, c, div, has_div, gscale);
-        else if constexpr (EXACT || !D::linear) val[JJ][KK] = eval_point<D, T, DM>(get, c, div, has_div, gscale);
-        else val[JJ][KK] = eval_fast<D, T>(get, c);
+        if constexpr (SHARE) {
+          ShareGet<D, T, CJ, CK, XJ, XK, JJ, KK> get{x, rin, cf};
+          const T u0 = rin[R].v[JJ][KK];
+          if constexpr (BATCHDIV) val[JJ][KK] = eval_share<D, T, 0>(get, u0, c, div, has_div);
+          else val[JJ][KK] = eval_share<D, T, DM>(get, u0, c, div, has_div);
+        } else {
+          RegGet<D, T, (NX > 0 ? NX : 1), CJ, CK, XJ, XK, JJ, KK> get{x, rin};
+          if constexpr (BATCHDIV) val[JJ][KK] = eval_point<D, T, 0>(get, c, div, has_div, gscale);
+          else if constexpr (EXACT || !D::linear) val[JJ][KK] = eval_point<D, T, DM>(get, c, div, has_div, gscale);
+          else val[JJ][KK] = eval_fast<D, T>(get, c);
+        }
       });
     });
     if constexpr (BATCHDIV) {
@@ -607,16 +699,18 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   // inside it: the ring rotation then costs no register moves.  The padding
   // iterations only produce planes past the segment, never stored.
   // 3D: a plane is CJ rows and the LDS exchange paces the loop; a shallow
-  // queue keeps the registers for the rings
-  constexpr int PF = prefetch_depth<UNR, CJ * CK * (int)(sizeof(T) / 4) * (DIM3 ? 3 : 1)>();
+  // queue keeps the registers for the rings.  The double-buffered radius-1
+  // stars are the exception: every wave of the workgroup meets at one barrier
+  // per plane, so with one plane in flight they would all wait on their loads
+  // together; they queue the whole unrolled body ahead.
+  constexpr int PF = prefetch_depth<UNR, CJ * CK * (int)(sizeof(T) / 4) *
+                                             (DIM3 && !star_exchange_db<D>(BT) ? 3 : 1)>();
   constexpr bool PF_ROT = UNR % PF == 0;  // queue slots rotate by renaming
   Plane<T, CJ, CK> pre[PF];
 #pragma unroll
   for (int f = 0; f < PF; f++) load_plane(pre[f], i0 - H + f);
   // radius-1 stars: edge rows of plane `pl` as stage slot `st`'s centre plane
   // of the next iteration, into exchange buffer `b`
-  constexpr bool STARX_DB = star_exchange_db<D>(BT);
-  constexpr size_t XBUF = (size_t)BT * NW * ERS * 64 * CK;  // elements per buffer
   auto publish = [&](const Plane<T, CJ, CK>& pl, int st, int b) {
     T* lds = reinterpret_cast<T*>(an5d_lds) + (size_t)b * XBUF;
 #pragma unroll
@@ -628,7 +722,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   if constexpr (STARX_DB) {
     // iteration 0 reads buffer 0: the (warm-up) centre planes of the rings
 #pragma unroll
-    for (int st = 0; st < BT; st++) publish(ring[st][R + 1], st, 0);
+    for (int st = 0; st < BT; st++) publish(SHARE ? scaled(ring[st][R + 1]) : ring[st][R + 1], st, 0);
   }
   const int nit_pad = (nit + UNR - 1) / UNR * UNR;
   for (int n0 = 0; n0 < nit_pad; n0 += UNR) {
@@ -639,23 +733,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
       const int slot = PF_ROT ? u % PF : 0;
       if constexpr (STARX_DB) {
         // the edge rows of iteration n were written during iteration n-1
-        // into buffer n&1; one barrier, then read the neighbours' rows
+        // into buffer n&1; one barrier, then each stage reads its rows
+        // (buffer n&1 is written again only after the next barrier)
         __syncthreads();
-        const T* lds = reinterpret_cast<const T*>(an5d_lds) + (size_t)(n & 1) * XBUF;
-        static_for<1, BT + 1>([&](auto sc) {
-          constexpr int S = decltype(sc)::value;
-          static_for<0, 2 * XJ>([&](auto hc) {
-            constexpr int HI = decltype(hc)::value;
-            constexpr int JR = HI < XJ ? HI - XJ : CJ + (HI - XJ);
-            constexpr int WOFF = JR >= 0 ? JR / CJ : -((-JR + CJ - 1) / CJ);
-            constexpr int RR = JR - WOFF * CJ;
-            constexpr int E = edge_index<CJ, XJ>(RR);
-            static_assert(E >= 0, "halo row must be published");
-            int ws = w + WOFF;
-            ws = ws < 0 ? 0 : (ws >= NW ? NW - 1 : ws);
-            load_run<T, CK>(xh[S - 1][HI], lds + ((((size_t)(S - 1) * NW + ws) * ERS + E) * 64 + lane) * CK);
-          });
-        });
+        xrd = n & 1;
       } else if constexpr (STARX) {
         // Every stage's centre plane this iteration is slot R+1 of its input
         // ring before the rotation: publish their edge rows, one barrier,
@@ -667,8 +748,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
             constexpr int JJ = decltype(jc)::value;
             constexpr int E = edge_index<CJ, XJ>(JJ);
             if constexpr (E >= 0)
-              store_run<T, CK>(lds + ((((size_t)(S - 1) * NW + w) * ERS + E) * 64 + lane) * CK,
-                               ring[S - 1][R + 1].v[JJ]);
+            {
+              T run[CK];
+#pragma unroll
+              for (int kk = 0; kk < CK; kk++)
+                run[kk] = SHARE ? cf * ring[S - 1][R + 1].v[JJ][kk] : ring[S - 1][R + 1].v[JJ][kk];
+              store_run<T, CK>(lds + ((((size_t)(S - 1) * NW + w) * ERS + E) * 64 + lane) * CK, run);
+            }
           });
         });
         __syncthreads();
@@ -696,12 +782,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         for (int f = 0; f + 1 < PF; f++) pre[f] = pre[f + 1];
       }
       load_plane(pre[PF_ROT ? slot : PF - 1], p_in + PF);
-      if constexpr (STARX_DB) publish(ring[0][P - 1], 0, (n + 1) & 1);
+      if constexpr (STARX_DB) publish(SHARE ? scaled(ring[0][P - 1]) : ring[0][P - 1], 0, (n + 1) & 1);
       static_for<1, BT + 1>([&](auto sc) {
         constexpr int S = decltype(sc)::value;
         Plane<T, CJ, CK> out;
         compute(ring[S - 1], out, p_in - S * R, sc);
-        if constexpr (STARX_DB && S < BT) publish(out, S, (n + 1) & 1);
+        if constexpr (STARX_DB && S < BT) publish(SHARE ? scaled(out) : out, S, (n + 1) & 1);
         if constexpr (S < BT) {
           // owned planes of stage S: n in [H + S*R, H + S*R + sle)
           if constexpr (MIDS) {
@@ -725,17 +811,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 
 // The division mode is fixed per launch: dispatch once, not per cell.
 template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT = true, bool MIDS = false,
-          int UNR = default_unroll<Def<ID>::rad>()>
+          int UNR = default_unroll<Def<ID>::rad>(), bool SHARE = false>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(SweepArgs<T> a) {
   if constexpr (!EXACT || !Def<ID>::linear) {
-    sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS>(a);  // no division in these forms
+    sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS, SHARE>(a);  // no division in these forms
   } else if constexpr (Def<ID>::divisor != 0) {
-    if (a.has_div == 2) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 2, UNR, MIDS>(a);
-    else if (a.has_div == 1) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 1, UNR, MIDS>(a);
-    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS>(a);
+    if (a.has_div == 2) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 2, UNR, MIDS, SHARE>(a);
+    else if (a.has_div == 1) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 1, UNR, MIDS, SHARE>(a);
+    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS, SHARE>(a);
   } else {
-    if (a.has_div == 0) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS>(a);
-    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, -1, UNR, MIDS>(a);
+    if (a.has_div == 0) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS, SHARE>(a);
+    else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, -1, UNR, MIDS, SHARE>(a);
   }
 }
 

@@ -17,7 +17,7 @@ struct Shape {
   int ck, cj, nw, btmax;
 };
 
-constexpr int kMaxShapes = 4;
+constexpr int kMaxShapes = 5;
 
 template <class D>
 constexpr bool is_box() {
@@ -57,6 +57,9 @@ constexpr Shape sweep_shape(int which) {
       // 192-cell rows (3 per lane): 512 = 3 tiles of 186 at bt 3 instead of
       // 5 of 122 (FP32, within 128 VGPRs)
       if (which == 3 && !f64) return Shape{3, 2, 16, 3};
+      // the same 192 x 32 tile in 8 waves of 4 rows: 256 VGPRs per wave at one
+      // workgroup per CU (room for the shared-product form's second ring)
+      if (which == 4 && !f64) return Shape{3, 4, 8, 3};
       return Shape{0, 0, 0, 0};
     }
     if constexpr (R == 2) {

@@ -36,7 +36,7 @@ class Config(ctypes.Structure):
                 ("mode", ctypes.c_int)]
 
 
-EXACT, FAST = 0, 1
+EXACT, FAST, EXACT_GENERAL = 0, 1, 2
 
 
 class Layout(ctypes.Structure):

@@ -56,9 +56,14 @@ typedef struct an5d_problem {
  * mode: AN5D_EXACT (0) reproduces the C loop bit for bit (source term order,
  * no contraction, correctly rounded division); AN5D_FAST (1) folds the
  * divisor into the coefficients and chains FMAs (results within the
- * tolerance checked by the tests, not bitwise). */
+ * tolerance checked by the tests, not bitwise).  In exact mode a radius-1
+ * star whose face taps all carry the same coefficient value forms each
+ * product c * u once per cell instead of once per use (the same rounded
+ * values in the same order: bitwise the same results);
+ * AN5D_EXACT_GENERAL (2) is exact mode without that form. */
 #define AN5D_EXACT 0
 #define AN5D_FAST 1
+#define AN5D_EXACT_GENERAL 2
 typedef struct an5d_config {
   int bt;
   int variant;

@@ -221,3 +221,39 @@ def test_fast_mode_within_tolerance(name, dt):
     floor = np.finfo(dt).tiny
     rel = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor))
     assert rel <= (1e-5 if dt == np.float32 else 1e-12), rel
+
+
+SHARE_CASES = [("star3d1r", np.float32), ("star3d1r", np.float64), ("star2d1r", np.float32),
+               ("star2d1r", np.float64), ("j2d5pt", np.float32), ("j2d5pt", np.float64)]
+
+
+@pytest.mark.parametrize("name,dt", SHARE_CASES)
+def test_shared_product_form_bitwise(name, dt):
+    """Radius-1 stars whose face taps carry one coefficient run the
+    shared-product kernels (each c * u formed once per cell): bit for bit
+    the oracle and the general kernels, for every shape and depth."""
+    d = O.info(name)
+    n = [37, 70, 300] if d["dim"] == 3 else [57, 700]
+    T = 11
+    ctr = [i for i, t in enumerate(O.taps(name)[0]) if not t[1:].any()][0]
+    coefs = np.full(d["ntaps"], 0.1)
+    coefs[ctr] = 0.4
+    div = 118.0 if d["divisor"] else 0.0
+    rng = np.random.default_rng(3)
+    u = rng.random([x + 2 * d["rad"] for x in n]).astype(dt)
+    A0 = np.stack([u, u])  # both buffers alike: the halo is one fixed boundary
+    ref = A0.copy()
+    O.run(name, ref, n, T, coefs=coefs, divisor=div)
+    dcode = 0 if dt == np.float32 else 1
+    for v in pyan5d.variants(name, dcode):
+        if v["kind"] != 0:
+            continue
+        for bt in sorted({1, 2, min(3, v["btmax"]), v["btmax"]}):
+            print(f"case shared-product {name} {np.dtype(dt).name} variant {v['index']} bt {bt}", flush=True)
+            A = A0.copy()
+            pyan5d.forward_host(name, A, n, T, coefs=coefs, divisor=div, bt=bt, variant=v["index"])
+            B = A0.copy()
+            pyan5d.forward_host(name, B, n, T, coefs=coefs, divisor=div, bt=bt, variant=v["index"],
+                                mode=pyan5d.EXACT_GENERAL)
+            assert np.array_equal(A[T % 2].view(np.uint8), ref[T % 2].view(np.uint8)), (v, bt)
+            assert np.array_equal(B[T % 2].view(np.uint8), ref[T % 2].view(np.uint8)), (v, bt)

@@ -192,6 +192,13 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
     lp.grid = dim3((unsigned)blocks);
     return true;
   }
+  // the sweep kernels address a plane (3D) or a row (2D) with 32-bit byte
+  // offsets from a per-plane base
+  const long long elem = v.dtype == kF64 ? 8 : 4;
+  if ((g.dim == 3 ? g.D1 * g.D2 : g.D2) * elem >= (1LL << 31)) {
+    err = "plane of 2 GiB or more: the register-streamed kernels address planes with 32-bit offsets";
+    return false;
+  }
   const long long SK = 64LL * v.ck - 2LL * bt * R;
   if (SK <= 0) { err = "bt too deep for this variant's tile width"; return false; }
   const long long ntk = cdiv(nk, SK);

@@ -102,6 +102,23 @@ __device__ __forceinline__ void store_run(T* p, const T (&src)[N]) {
   __builtin_memcpy(p, &t, sizeof(t));
 }
 
+// Buffer-resource loads: a wave-uniform base in the descriptor, a
+// wave-uniform byte offset (soffset) and a per-lane 32-bit byte offset
+// (voffset), so the streamed plane loads need no per-lane 64-bit address
+// arithmetic.  The range field is left at its maximum (no clamping: every
+// access is in bounds by construction); 0x00020000 is the gfx9-family word 3
+// (32-bit data format, which untyped accesses ignore).  aux bit 1 =
+// non-temporal.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0xffffffff, 0x00020000);
+}
+template <bool NT, typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+  constexpr int aux = NT ? 2 : 0;
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, aux));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, aux));
+}
+
 // Global accesses of 4-byte-aligned runs; NT: non-temporal (streaming) load
 // or store.  The non-temporal hint pays where a launch is bandwidth-bound
 // (one or two fused steps) and costs where it is compute-bound (the halo
@@ -459,7 +476,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
     jin[jj] = !DIM3 || (j >= R && j < a.D1 - R);
     jst[jj] = !DIM3 || (j >= jv0 && j < jv1);
   }
-  const bool kfull = kl >= 0 && kl + CK <= a.D2;
   // every cell of this wave interior along dims 1-2: no Dirichlet select
   bool cells_in = true;
 #pragma unroll
@@ -494,25 +510,30 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   // clamped into the grid (the stand-in values only reach cells outside it,
   // which are never stored).  Loads are unconditional, so a queued plane never
   // merges with an older value, which would wait for the load at once.
-  const bool wave_kfull = __builtin_amdgcn_ballot_w64(!kfull) == 0;
+  // Loads go through buffer descriptors: the plane's base (moved down by H
+  // cells) in the descriptor, the row's byte offset in soffset and the cell's
+  // (column k + H, clamped into the row) in voffset; all 32-bit (the host
+  // keeps a plane below 2 GiB) and none computed in the loop.  Stores keep
+  // the 64-bit address form, which lets a lane write its run with one
+  // multi-word store (measured faster than word-wise buffer stores).
+  constexpr unsigned SZ = sizeof(T);
+  unsigned vclamp[CK];
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) {
+    int k = kl + kk;
+    k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+    vclamp[kk] = (unsigned)(k + H) * SZ;
+  }
   auto load_plane = [&](Plane<T, CJ, CK>& pl, int p) {
     p = p < 0 ? 0 : (p >= a.D0 ? a.D0 - 1 : p);
-    const T* base = a.src + (long long)p * a.s0;
+    const auto rs = make_rsrc(a.src + (long long)p * a.s0 - H);
 #pragma unroll
     for (int jj = 0; jj < CJ; jj++) {
       int j = jw + jj;
       if (DIM3) j = j < 0 ? 0 : (j >= a.D1 ? a.D1 - 1 : j);
-      const T* row = base + (DIM3 ? (long long)j * a.s1 : 0ll);
-      if (wave_kfull) {
-        gload_run<STREAM, T, CK>(pl.v[jj], row + kl);
-      } else {
+      const int so = DIM3 ? j * a.s1 * (int)SZ : 0;
 #pragma unroll
-        for (int kk = 0; kk < CK; kk++) {
-          int k = kl + kk;
-          k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
-          pl.v[jj][kk] = row[k];
-        }
-      }
+      for (int kk = 0; kk < CK; kk++) pl.v[jj][kk] = bload<STREAM, T>(rs, vclamp[kk], so);
     }
   };
 
@@ -705,10 +726,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
   // together; they queue the whole unrolled body ahead.
   constexpr int PF = prefetch_depth<UNR, CJ * CK * (int)(sizeof(T) / 4) *
                                              (DIM3 && !star_exchange_db<D>(BT) ? 3 : 1)>();
-  constexpr bool PF_ROT = UNR % PF == 0;  // queue slots rotate by renaming
-  Plane<T, CJ, CK> pre[PF];
-#pragma unroll
-  for (int f = 0; f < PF; f++) load_plane(pre[f], i0 - H + f);
+  // The loop body holds U iterations.  A loaded plane lives P + PF
+  // iterations (queue, then input ring); when that exceeds U it crosses two
+  // loop back-edges and the register allocator copies it (waiting for the
+  // load) at the end of the body.  Radius-1 kernels of moderate body size
+  // (BT * CJ * CK <= 36 cell-stages per plane) unroll 2*UNR iterations so
+  // every value crosses at most one; longer bodies would exceed what the
+  // compiler fully unrolls or cost occupancy, and the wider radii keep UNR.
+  constexpr int U = R == 1 && UNR == P && P + PF > UNR && PF <= P && BT * CJ * CK <= 36 ? 2 * UNR : UNR;
+  constexpr bool PF_ROT = U % PF == 0;  // queue slots rotate by renaming
   // radius-1 stars: edge rows of plane `pl` as stage slot `st`'s centre plane
   // of the next iteration, into exchange buffer `b`
   auto publish = [&](const Plane<T, CJ, CK>& pl, int st, int b) {
@@ -724,10 +750,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 #pragma unroll
     for (int st = 0; st < BT; st++) publish(SHARE ? scaled(ring[st][R + 1]) : ring[st][R + 1], st, 0);
   }
-  const int nit_pad = (nit + UNR - 1) / UNR * UNR;
-  for (int n0 = 0; n0 < nit_pad; n0 += UNR) {
+  const int nit_pad = (nit + U - 1) / U * U;
+  Plane<T, CJ, CK> pre[PF];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) {
+  for (int f = 0; f < PF; f++) load_plane(pre[f], i0 - H + f);
+  // the U iterations from n0
+  auto block = [&](int n0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
       const int n = n0 + u;
       const int p_in = i0 - H + n;
       const int slot = PF_ROT ? u % PF : 0;
@@ -806,7 +836,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
         }
       });
     }
-  }
+  };
+  for (int n0 = 0; n0 < nit_pad; n0 += U) block(n0);
 }
 
 // The division mode is fixed per launch: dispatch once, not per cell.

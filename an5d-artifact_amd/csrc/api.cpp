@@ -301,6 +301,7 @@ struct DeviceCache {
   std::mutex mu;
   std::map<std::pair<int, std::string>, void*> coefs;  // (device, bytes) -> device copy
   std::map<int, std::pair<void*, size_t>> grids;       // device -> staging grid buffer
+  std::map<int, std::mutex> grid_users;                // device -> holder of its staging grid
 };
 static DeviceCache& cache() {
   static DeviceCache* c = new DeviceCache();
@@ -329,11 +330,18 @@ int device_coefs(const std::vector<unsigned char>& bytes, const void** out) {
   return AN5D_OK;
 }
 
-int device_grid(size_t bytes, void** out) {
+int device_grid(size_t bytes, void** out, std::unique_lock<std::mutex>& hold) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   DeviceCache& c = cache();
+  std::mutex* user;
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    user = &c.grid_users[dev];  // map nodes stay put
+  }
+  // one host call at a time uses (and may regrow) a device's staging grid
+  hold = std::unique_lock<std::mutex>(*user);
   std::lock_guard<std::mutex> lk(c.mu);
   auto& slot = c.grids[dev];
   if (slot.second < bytes) {
@@ -496,6 +504,7 @@ extern "C" int an5d_variant_get_info(int stencil, int dtype, int variant, an5d_v
   const Variant* v = vs[variant];
   out->kind = v->kind; out->ck = v->ck; out->cj = v->cj; out->nw = v->nw;
   out->btmax = v->btmax; out->lds_bytes = (long long)v->lds[v->btmax];
+  out->regcap = v->regcap;
   return AN5D_OK;
 }
 
@@ -607,7 +616,8 @@ extern "C" int an5d_forward_host(const an5d_problem* p, void* A, const an5d_conf
   const size_t esz = p->dtype == AN5D_FLOAT64 ? 8 : 4;
   an5d_layout L = padded_layout(g);
   void* dA = nullptr;
-  int rc = device_grid((size_t)2 * L.buf_stride * esz, &dA);
+  std::unique_lock<std::mutex> hold;  // the staging grid is ours until we return
+  int rc = device_grid((size_t)2 * L.buf_stride * esz, &dA, hold);
   if (rc) return rc;
   const size_t rows = (size_t)g.D0 * g.D1;
   hipError_t e;

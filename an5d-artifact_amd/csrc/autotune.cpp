@@ -250,6 +250,17 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
   // that field, so all of them see the same data.
   void* snap = nullptr;
   const size_t grid_bytes = (size_t)2 * L.buf_stride * esz;
+  // a caller's grid is put back as it was when the search ends
+  std::vector<unsigned char> caller_copy;
+  if (!owned) {
+    try {
+      caller_copy.resize(grid_bytes);
+    } catch (...) {
+      return fail(AN5D_E_ALLOC, "no host memory to keep the caller's grid during the search");
+    }
+    e = hipMemcpy(caller_copy.data(), buf, grid_bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(autotune keep grid)");
+  }
   {
     int warm = p->timesteps / 2;
     if (warm > 512) warm = 512;
@@ -261,6 +272,8 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
       int rc0 = an5d_forward_device(&w, buf, &L, &wc, nullptr);
       if (rc0) {
         if (owned) (void)hipFree(owned);
+        if (!caller_copy.empty())
+          (void)hipMemcpy(buf, caller_copy.data(), grid_bytes, hipMemcpyHostToDevice);
         return rc0;
       }
     }
@@ -311,6 +324,10 @@ extern "C" int an5d_autotune(const an5d_problem* p, void* dA, const an5d_layout*
   (void)hipEventDestroy(e1);
   if (snap) (void)hipFree(snap);
   if (owned) (void)hipFree(owned);
+  if (!caller_copy.empty()) {
+    e = hipMemcpy(buf, caller_copy.data(), grid_bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipMemcpy(autotune restore grid)");
+  }
   if (rc) return rc;
   const Cand* win = &cands[0];
   for (const Cand& c : cands)
@@ -349,10 +366,16 @@ extern "C" int an5d_tune_save(const char* path) {
   const std::string p = path ? path : default_path();
   FILE* f = fopen(p.c_str(), "w");
   if (!f) return fail(AN5D_E_INVALID, "cannot write " + p);
-  fprintf(f, "# stencil dtype mode n0 n1 n2 variant bt sl gstencil_per_s (autotuner results)\n");
-  for (const Entry& e : t.rows)
-    fprintf(f, "%s %d %d %lld %lld %lld %d %d %d %.1f\n", stencil_rt(e.stencil).name.c_str(), e.dtype,
-            e.mode, e.n[0], e.n[1], e.n[2], e.variant, e.bt, e.sl, e.gstencil);
+  fprintf(f, "# stencil dtype mode n0 n1 n2 variant bt sl gstencil_per_s regcap (autotuner results; regcap:\n"
+             "# registers per lane the chosen kernel was compiled to, 0 = no cap; informational)\n");
+  for (const Entry& e : t.rows) {
+    auto vs = variants_of(e.stencil, e.dtype);
+    const int cap = e.variant >= 0 && e.variant < (int)vs.size() && e.bt * 2 > vs[e.variant]->btmax
+                        ? vs[e.variant]->regcap
+                        : 0;
+    fprintf(f, "%s %d %d %lld %lld %lld %d %d %d %.1f %d\n", stencil_rt(e.stencil).name.c_str(), e.dtype,
+            e.mode, e.n[0], e.n[1], e.n[2], e.variant, e.bt, e.sl, e.gstencil, cap);
+  }
   fclose(f);
   return AN5D_OK;
 }

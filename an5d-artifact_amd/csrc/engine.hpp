@@ -57,6 +57,7 @@ using LaunchFn = hipError_t (*)(const LaunchArgs&, dim3 grid, dim3 block, size_t
 struct Variant {
   int stencil, dtype, kind;
   int ck, cj, nw, btmax;
+  int regcap;              // register cap per lane of the deeper blocks (0: none)
   size_t lds[kMaxBT + 1];  // dynamic LDS bytes per workgroup, by bt
   LaunchFn fn[kMaxBT + 1];       // fn[bt], bt in [1, btmax], exact mode
   LaunchFn fn_fast[kMaxBT + 1];  // fast (FMA) mode, same shapes

@@ -38,18 +38,23 @@ SweepArgs<U> to_args(const LaunchArgs& a) {
 constexpr int kUnroll = default_unroll<Def<SID>::rad>();
 constexpr bool kShare = share_capable<Def<SID>>();
 
-template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS = false, bool SHARE = false>
+// The kernel of one instance: WPE > 0 selects the register-capped form.
+template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS, bool SHARE, int WPE>
+constexpr auto sweep_fn() {
+  if constexpr (WPE > 0) return &sweep_kernel_cap<SID, T, BT, CK, CJ, NW, EXACT, WPE>;
+  else return &sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS, kUnroll, SHARE>;
+}
+
+template <int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS = false, bool SHARE = false, int WPE = 0>
 hipError_t launch_sweep(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  constexpr auto kern = sweep_fn<BT, CK, CJ, NW, EXACT, MIDS, SHARE, WPE>();
   if (lds > 65536) {
     // gfx950 has 160 KB of LDS per CU
     static std::atomic<int> done[kMaxDevices];
-    const hipError_t e = lds_opt_in(
-        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS, kUnroll, SHARE>), lds,
-        done);
+    const hipError_t e = lds_opt_in(reinterpret_cast<const void*>(kern), lds, done);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, MIDS, kUnroll, SHARE>), grid, block, lds, st,
-                     to_args<T>(a));
+  hipLaunchKernelGGL(kern, grid, block, lds, st, to_args<T>(a));
   return hipGetLastError();
 }
 
@@ -68,11 +73,10 @@ int share_scratch_once() {
 
 // Resident workgroups per CU (0 if the query fails); computed once, on the
 // first device that asks (every device is a gfx950 with the same resources).
-template <int BT, int CK, int CJ, int NW, bool EXACT, bool SHARE = false>
+template <int BT, int CK, int CJ, int NW, bool EXACT, bool SHARE = false, int WPE = 0>
 int occupancy_once() {
   static const int n = [] {
-    const void* k =
-        reinterpret_cast<const void*>(&sweep_kernel<SID, T, BT, CK, CJ, NW, EXACT, false, kUnroll, SHARE>);
+    const void* k = reinterpret_cast<const void*>(sweep_fn<BT, CK, CJ, NW, EXACT, false, SHARE, WPE>());
     const size_t lds = sweep_lds_bytes<SID, T, CK, CJ, NW>(BT);
     static std::atomic<int> done[kMaxDevices];
     if (lds > 65536 && lds_opt_in(k, lds, done) != hipSuccess) return 0;
@@ -82,11 +86,12 @@ int occupancy_once() {
   }();
   return n;
 }
-template <int BT, int CK, int CJ, int NW>
+template <int BT, int CK, int CJ, int NW, int WPE = 0>
 int occupancy_sweep(int mode) {
-  if constexpr (kShare)
+  if constexpr (kShare && WPE == 0)
     if (mode == 2) return occupancy_once<BT, CK, CJ, NW, true, true>();
-  return mode == 1 ? occupancy_once<BT, CK, CJ, NW, false>() : occupancy_once<BT, CK, CJ, NW, true>();
+  return mode == 1 ? occupancy_once<BT, CK, CJ, NW, false, false, WPE>()
+                   : occupancy_once<BT, CK, CJ, NW, true, false, WPE>();
 }
 
 template <int CK>
@@ -164,6 +169,38 @@ void add_sweep(std::vector<Variant>& out) {
   }
 }
 
+// Register-capped twin of shape W (the cap as a search dimension): workgroups
+// of 4 waves are capped at 2 waves per SIMD (256 registers) where the rings
+// are wide (FP64, radius >= 2) and at 4 (128) otherwise, workgroups of 8 at 4
+// (128); 16-wave workgroups are at 128 already and have no twin.  The deeper
+// half of the temporal blocks (where the registers run out) takes the capped
+// kernels; the shallower ones, and the intermediate-stage launches, share the
+// uncapped shape's kernels.
+template <int W>
+void add_capped(std::vector<Variant>& out) {
+  constexpr Shape sh = fitted_shape<W>();
+  if constexpr (sh.btmax > 0 && sh.nw <= 8) {
+    constexpr int wpe = sh.nw == 4 && (AN5D_F64 || Def<SID>::rad >= 2) ? 2 : 4;
+    Variant v{};
+    v.stencil = SID;
+    v.dtype = AN5D_F64 ? kF64 : kF32;
+    v.kind = kSweep;
+    v.ck = sh.ck; v.cj = sh.cj; v.nw = sh.nw; v.btmax = sh.btmax;
+    v.regcap = 512 / wpe;
+    static_for<1, sh.btmax + 1>([&](auto bc) {
+      constexpr int BT = decltype(bc)::value;
+      constexpr int C = 2 * BT > sh.btmax ? wpe : 0;
+      v.lds[BT] = sweep_lds_bytes<SID, T, sh.ck, sh.cj, sh.nw>(BT);
+      v.fn[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true, false, false, C>;
+      v.fn_fast[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, false, false, false, C>;
+      v.occ[BT] = &occupancy_sweep<BT, sh.ck, sh.cj, sh.nw, C>;
+      if constexpr (Def<SID>::dim == 2 && BT >= 2 && BT <= kMaxMidBT)
+        v.fn_mid[BT] = &launch_sweep<BT, sh.ck, sh.cj, sh.nw, true, true>;
+    });
+    out.push_back(v);
+  }
+}
+
 }  // namespace
 
 #define AN5D_CAT2(a, b, c) a##b##_##c
@@ -200,6 +237,13 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
     pv.occ[1] = &occupancy_plane<CK, CJ, NW>;
     out.push_back(pv);
   }
+  // register-capped twins of the sweep shapes, after the other variants so
+  // the indices of those stay put
+  add_capped<0>(out);
+  add_capped<1>(out);
+  add_capped<2>(out);
+  add_capped<3>(out);
+  add_capped<4>(out);
 }
 
 }  // namespace an5d

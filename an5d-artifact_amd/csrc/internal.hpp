@@ -1,6 +1,7 @@
 // Declarations shared by the host translation units of liban5d.
 #pragma once
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,7 +52,9 @@ bool coefs_shareable(const an5d_problem* p, const std::vector<unsigned char>& by
 bool build_coefs(const an5d_problem* p, std::vector<unsigned char>& bytes, int& has_div,
                  double& gscale, std::string& err, int mode = 0);
 int device_coefs(const std::vector<unsigned char>& bytes, const void** out);
-int device_grid(size_t bytes, void** out);
+// The current device's staging grid of at least `bytes`; `hold` keeps other
+// host calls on that device from using or regrowing it until released.
+int device_grid(size_t bytes, void** out, std::unique_lock<std::mutex>& hold);
 int run_schedule(const an5d_problem* p, void* dA, const an5d_layout* layout,
                  const an5d_config* cfg, int steps_only, int c0_first, hipStream_t st);
 double sb_time_ms();

@@ -841,9 +841,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T>& a, int tile = -1)
 }
 
 // The division mode is fixed per launch: dispatch once, not per cell.
-template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT = true, bool MIDS = false,
-          int UNR = default_unroll<Def<ID>::rad>(), bool SHARE = false>
-__global__ __launch_bounds__(64 * NW) void sweep_kernel(SweepArgs<T> a) {
+template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT, bool MIDS, int UNR, bool SHARE>
+__device__ __forceinline__ void sweep_dispatch(const SweepArgs<T>& a) {
   if constexpr (!EXACT || !Def<ID>::linear) {
     sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS, SHARE>(a);  // no division in these forms
   } else if constexpr (Def<ID>::divisor != 0) {
@@ -854,6 +853,22 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(SweepArgs<T> a) {
     if (a.has_div == 0) sweep_body<ID, T, BT, CK, CJ, NW, EXACT, 0, UNR, MIDS, SHARE>(a);
     else sweep_body<ID, T, BT, CK, CJ, NW, EXACT, -1, UNR, MIDS, SHARE>(a);
   }
+}
+
+template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT = true, bool MIDS = false,
+          int UNR = default_unroll<Def<ID>::rad>(), bool SHARE = false>
+__global__ __launch_bounds__(64 * NW) void sweep_kernel(SweepArgs<T> a) {
+  sweep_dispatch<ID, T, BT, CK, CJ, NW, EXACT, MIDS, UNR, SHARE>(a);
+}
+
+// The same sweep with a register cap: at least WPE waves per SIMD, so at
+// most 512 / WPE registers per lane (the compiler spills what does not fit).
+// The cap trades spills for occupancy, as nvcc's --maxrregcount does for the
+// reference's binaries (make.sh:29-31); the tuner measures both.
+template <int ID, typename T, int BT, int CK, int CJ, int NW, bool EXACT, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep_kernel_cap(
+    SweepArgs<T> a) {
+  sweep_dispatch<ID, T, BT, CK, CJ, NW, EXACT, false, default_unroll<Def<ID>::rad>(), false>(a);
 }
 
 // One step without temporal blocking: every thread evaluates CK adjacent

@@ -51,7 +51,8 @@ class StencilInfo(ctypes.Structure):
 
 class VariantInfo(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("ck", ctypes.c_int), ("cj", ctypes.c_int),
-                ("nw", ctypes.c_int), ("btmax", ctypes.c_int), ("lds_bytes", ctypes.c_longlong)]
+                ("nw", ctypes.c_int), ("btmax", ctypes.c_int), ("lds_bytes", ctypes.c_longlong),
+                ("regcap", ctypes.c_int)]
 
 
 class BatchGrad(ctypes.Structure):
@@ -164,7 +165,7 @@ def variants(name, dtype=FLOAT32):
         vi = VariantInfo()
         _check(lib().an5d_variant_get_info(sid, dtype, v, ctypes.byref(vi)))
         out.append(dict(index=v, kind=vi.kind, ck=vi.ck, cj=vi.cj, nw=vi.nw, btmax=vi.btmax,
-                        lds_bytes=vi.lds_bytes))
+                        lds_bytes=vi.lds_bytes, regcap=vi.regcap))
     return out
 
 
@@ -521,12 +522,14 @@ class BatchAdjoint(Batch):
         self._info()
 
 
-def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None, timesteps=1):
+def autotune(name, n, dtype=np.float32, mode=EXACT, max_measured=24, tensor=None, timesteps=1, coefs=None,
+             divisor=-1.0):
     """Rank configurations with the model, time the best on the GPU, record
     the winner.  `timesteps`: the length of the run being tuned for (the
-    candidates are timed on the field halfway through it).  Returns (config
-    dict, GStencil/s)."""
-    p, keep = make_problem(name, n, timesteps, dtype)
+    candidates are timed on the field halfway through it); `coefs` /
+    `divisor`: the run's coefficients (they decide, e.g., whether the
+    shared-product form applies).  Returns (config dict, GStencil/s)."""
+    p, keep = make_problem(name, n, timesteps, dtype, coefs, divisor)
     best = Config()
     gs = ctypes.c_double()
     ptr, lay = None, None

@@ -3,8 +3,9 @@
     python -m pyan5d.tune [--set bench|all] [--out FILE] [--max-measured K]
 
 For every (stencil, dtype, size, mode, run length) of the set, an5d_autotune
-ranks all (kernel shape, bt, sl) candidates with the analytic model, times the
-best K on the GPU (on the field halfway through the run) and keeps the
+ranks all (kernel shape, register cap, bt, sl) candidates with the analytic
+model, times the best K on the GPU (on the field halfway through the run,
+with the benchmark's coefficients: pyan5d.presets "baseline") and keeps the
 fastest; the table is then written to FILE (default:
 the file the library loads, an5d-artifact_amd/tuned.tsv).  This replaces the
 reference's Excel model + top-5 compile/run scripts
@@ -20,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import EXACT, FAST, autotune, info, tune_clear, tune_load, tune_save
+from . import EXACT, FAST, autotune, info, presets, tune_clear, tune_load, tune_save
 
 STENCILS_2D = ["star2d1r", "star2d2r", "star2d3r", "star2d4r", "box2d1r", "box2d2r", "box2d3r",
                "box2d4r", "j2d5pt", "j2d9pt", "j2d9pt-gol", "gradient2d"]
@@ -72,7 +73,9 @@ def main(argv=None):
         info(name)  # validates the name
         for mode in modes:
             t0 = time.perf_counter()
-            cfg, gs = autotune(name, n, dt, mode=mode, max_measured=args.max_measured, timesteps=steps)
+            coefs, div = presets.coefficients(name, "baseline")
+            cfg, gs = autotune(name, n, dt, mode=mode, max_measured=args.max_measured, timesteps=steps,
+                               coefs=coefs, divisor=div)
             print(f"[tune] {name} {np.dtype(dt).name} {'x'.join(map(str, n))} "
                   f"{'fast' if mode == FAST else 'exact'}: variant {cfg['variant']} bt {cfg['bt']} "
                   f"sl {cfg['sl']} -> {gs:.1f} GStencil/s ({time.perf_counter() - t0:.1f} s)", flush=True)

@@ -62,6 +62,7 @@ def demangle(names):
 
 FIELDS = {
     "sweep_kernel": ["bt", "ck", "cj", "nw", "exact", "mids", "unr", "share"],
+    "sweep_kernel_cap": ["bt", "ck", "cj", "nw", "exact", "waves_per_eu"],
     "plane_kernel": ["bt", "ck", "cj", "nw", "exact"],
     "direct_kernel": ["ck"],
     "adjoint_kernel": ["bt", "ck", "ckpt"],

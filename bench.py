@@ -32,6 +32,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "GStencil/s & % of bt-scaled HBM roofline; fwd+adj and batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9  # unfused FP32/FP64 vector lane-ops per second
 
 WORKLOADS = {
     "star3d1r-f32": dict(stencil="star3d1r", dtype=np.float32, n=[512, 512, 512], T=1000,
@@ -254,7 +255,7 @@ def bench_batched(torch, pyan5d, presets, reps=2):
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
     t0 = time.perf_counter()
     for name, n, T, t, coefs, div in items:
-        pyan5d.autotune(name, n, max_measured=6, timesteps=T)  # into the in-memory table
+        pyan5d.autotune(name, n, max_measured=6, timesteps=T, coefs=coefs, divisor=div)  # in-memory table
     log(f"[bench]   per-grid autotuning of {len(items)} grids: {time.perf_counter() - t0:.1f} s")
 
     def timed(fn, r):
@@ -457,7 +458,12 @@ def main():
                    "variant": variants[variant], "sl": sl,
                    "gflops": round(gstencil * d["fpp"], 2),
                    "bt_scaled_hbm_roof_gstencil": HBM_PEAK_GBS * bt / (2 * esz),
-                   "frac_of_bt_scaled_roof": round(gstencil / (HBM_PEAK_GBS * bt / (2 * esz)), 4)},
+                   "frac_of_bt_scaled_roof": round(gstencil / (HBM_PEAK_GBS * bt / (2 * esz)), 4),
+                   # the fused sweep is issue-bound: the reference's FLOPs per update
+                   # as unfused vector ops (FP32 and FP64: 256 CUs x 4 SIMDs x 16
+                   # lanes x 2.4 GHz = 39.3 T lane-ops/s) bound it from above
+                   "valu_roof_gstencil": round(VALU_LANE_OPS / d["fpp"] / 1e9, 1),
+                   "frac_of_valu_roof": round(gstencil / (VALU_LANE_OPS / d["fpp"] / 1e9), 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "error": errs,

@@ -89,6 +89,10 @@ typedef struct an5d_variant_info {
                (3D stars of radius 2-4) */
   int ck, cj, nw, btmax;
   long long lds_bytes;
+  int regcap; /* registers per lane the compiler was held to in the deeper
+                 temporal blocks (bt > btmax/2), 0: no cap beyond the
+                 workgroup size's; the --maxrregcount analogue
+                 (make.sh:29-31) */
 } an5d_variant_info;
 
 /* ---- registry ------------------------------------------------------------
@@ -124,7 +128,9 @@ int an5d_kernel_stencil(int stencil, int dtype, void* A1, int compsize, int time
                         int scop, double* ms);
 
 /* Same as the GPU branch above for any problem (rectangular grids, other
- * coefficients) and tiling.  A: host array, reference layout. */
+ * coefficients) and tiling.  A: host array, reference layout.  Thread-safe:
+ * calls from several threads on one device take turns on its staging grid
+ * (each call holds it from the upload to the download). */
 int an5d_forward_host(const an5d_problem* prob, void* A, const an5d_config* cfg, double* ms);
 
 /* Device-resident form: dA holds both buffers in `layout` (NULL: reference
@@ -257,8 +263,12 @@ int an5d_batch_destroy(an5d_batch* batch);
  * analytic model (an5d_model_estimate), times the best max_measured of them
  * on the GPU (dA: the caller's device grid, or NULL for a scratch grid) and
  * records the winner in the table that the forward entry points consult
- * when cfg->variant < 0.  an5d_tune_save/load persist the table as text
- * (default file: an5d-artifact_amd/tuned.tsv, or $AN5D_TUNE_FILE). */
+ * when cfg->variant < 0.  The shapes include register-capped twins
+ * (an5d_variant_info.regcap), so the cap is searched like the reference's
+ * --maxrregcount builds.  A caller's grid is kept in host memory during the
+ * search and written back before the call returns.  an5d_tune_save/load
+ * persist the table as text (default file: an5d-artifact_amd/tuned.tsv, or
+ * $AN5D_TUNE_FILE; its last column records the chosen kernel's cap). */
 int an5d_autotune(const an5d_problem* prob, void* dA, const an5d_layout* layout, int mode,
                   int max_measured, an5d_config* best, double* best_gstencil);
 int an5d_model_estimate(const an5d_problem* prob, const an5d_config* cfg, double* gstencil);

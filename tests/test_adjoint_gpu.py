@@ -60,6 +60,41 @@ def test_adjoint_vs_oracle(name, dt, bt):
     assert np.max(np.abs(gc - ogc)) <= tol * max(1e-30, np.max(np.abs(ogc)))
 
 
+def division_cases():
+    out = []
+    for name in ("j2d5pt", "j2d9pt", "j2d9pt-gol"):
+        for dt in (np.float32, np.float64):
+            for bt in sorted({1, adj_btmax(name, dt)}):
+                out.append((name, dt, bt, "reference"))
+        out.append((name, np.float32, 2, "baseline") if name == "j2d5pt" else None)
+    return [c for c in out if c]
+
+
+@pytest.mark.parametrize("name,dt,bt,preset", division_cases())
+def test_adjoint_with_division_vs_oracle(name, dt, bt, preset):
+    """The division path: the reference's own coefficients over 118
+    (divisor -1: the folded taps s_q c_q / div, the coefficient gradient
+    divided by div, the checkpointed forward's corrected division) and, for
+    j2d5pt, the benchmark preset (S = 12.3, / 118)."""
+    from pyan5d import presets
+    n = [37, 300]
+    T = 2 * bt + 1
+    rng = np.random.default_rng(7 + bt)
+    shape = tuple(x + 2 * O.info(name)["rad"] for x in n)
+    u0 = rng.random(shape).astype(dt)
+    target = rng.random(shape).astype(dt)
+    c, div = presets.coefficients(name, preset)
+    assert div != 0.0
+    print(f"case adjoint-division {name} {np.dtype(dt).name} bt={bt} T={T} preset={preset}", flush=True)
+    loss, g, gc, _, _ = pyan5d.adjoint_host(name, u0, target, n, T, coefs=c, divisor=div, bt=bt)
+    oloss, og, ogc = O.adjoint(name, u0.astype(np.float64), target.astype(np.float64), n, T,
+                               coefs=c, divisor=div)
+    tol = 1e-10 if dt == np.float64 else 2e-4
+    assert abs(loss - oloss) <= tol * max(1.0, abs(oloss))
+    assert np.max(np.abs(g - og)) <= tol * max(1e-30, np.max(np.abs(og)))
+    assert np.max(np.abs(gc - ogc)) <= tol * max(1e-30, np.max(np.abs(ogc)))
+
+
 @pytest.mark.parametrize("name", ["j2d9pt-gol", "j2d9pt", "gradient2d", "j2d5pt"])
 def test_adjoint_central_differences_fp64(name):
     """FD on a 64x64 FP64 instance through the GPU forward."""

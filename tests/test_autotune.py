@@ -27,7 +27,30 @@ def test_tune_table_roundtrip(tmp_path):
     out = str(tmp_path / "u.tsv")
     pyan5d.tune_save(out)
     lines = [l for l in open(out) if not l.startswith("#")]
-    assert lines == ["j2d5pt 0 0 1000 1000 0 0 3 64 1234.5\n"]
+    # the last column is the chosen kernel's register cap (variant 0 of
+    # j2d5pt FP32 has none); it is informational and ignored on load
+    assert lines == ["j2d5pt 0 0 1000 1000 0 0 3 64 1234.5 0\n"]
+    pyan5d.tune_clear()
+
+
+def test_tune_table_records_register_cap(tmp_path):
+    """The cap column: a capped twin's deep block reports its cap, its
+    shallow blocks (which share the uncapped kernels) report none."""
+    vs = pyan5d.variants("j2d5pt", pyan5d.FLOAT32)
+    capped = [v for v in vs if v["regcap"] > 0]
+    assert capped and all(v["kind"] == 0 for v in capped)
+    v = capped[0]
+    base = [u for u in vs if u["regcap"] == 0 and (u["ck"], u["cj"], u["nw"]) == (v["ck"], v["cj"], v["nw"])]
+    assert base and base[0]["btmax"] == v["btmax"] and base[0]["index"] < v["index"]
+    path = tmp_path / "t.tsv"
+    path.write_text(f"j2d5pt 0 0 1000 1000 0 {v['index']} {v['btmax']} 64 1.0\n"
+                    f"j2d5pt 0 0 2000 2000 0 {v['index']} 1 64 1.0\n")
+    pyan5d.tune_clear()
+    pyan5d.tune_load(str(path))
+    out = str(tmp_path / "u.tsv")
+    pyan5d.tune_save(out)
+    caps = [int(l.split()[-1]) for l in open(out) if not l.startswith("#")]
+    assert caps == [v["regcap"], 0]
     pyan5d.tune_clear()
 
 
@@ -80,3 +103,19 @@ def test_tune_workload_sets_cover_every_stencil():
     assert len(names) == pyan5d.lib().an5d_stencil_count()
     bench = tune.workloads("bench")
     assert ("star3d1r", np.float32, [512, 512, 512], (pyan5d.EXACT, pyan5d.FAST), 1000) in bench
+
+
+@pytest.mark.gpu
+def test_autotune_leaves_the_callers_grid_as_it_was(torch_cuda):
+    torch = torch_cuda
+    print("case autotune caller grid j2d5pt 300x500", flush=True)
+    n = [300, 500]
+    host = pyan5d.init_grid("j2d5pt", n, np.float32, mode=1)
+    t = torch.from_numpy(host).cuda()
+    before = t.clone()
+    pyan5d.tune_clear()
+    cfg, gs = pyan5d.autotune("j2d5pt", n, max_measured=4, tensor=t, timesteps=40)
+    torch.cuda.synchronize()
+    assert gs > 0
+    assert torch.equal(t.view(torch.uint8), before.view(torch.uint8))
+    pyan5d.tune_clear()

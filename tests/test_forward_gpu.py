@@ -257,3 +257,37 @@ def test_shared_product_form_bitwise(name, dt):
                                 mode=pyan5d.EXACT_GENERAL)
             assert np.array_equal(A[T % 2].view(np.uint8), ref[T % 2].view(np.uint8)), (v, bt)
             assert np.array_equal(B[T % 2].view(np.uint8), ref[T % 2].view(np.uint8)), (v, bt)
+
+
+def test_forward_host_from_several_threads():
+    """Host calls from several threads on one device, with grids of
+    different sizes (the staging grid grows while others are running): every
+    result equals the oracle's."""
+    import threading
+    cases = [([300, 700], 9), ([900, 1200], 7), ([150, 2000], 5), ([1100, 900], 8)]
+    print("case threads forward_host j2d5pt x4", flush=True)
+    # both buffers hold the same field (the halo is read from either one)
+    inputs = []
+    for i, (n, _) in enumerate(cases):
+        u = np.random.default_rng(i).random((n[0] + 2, n[1] + 2)).astype(np.float32)
+        inputs.append(np.stack([u, u]))
+    outs = [a.copy() for a in inputs]
+    errors = []
+
+    def work(i):
+        try:
+            n, T = cases[i]
+            pyan5d.forward_host("j2d5pt", outs[i], n, T)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (n, T), a, o in zip(cases, inputs, outs):
+        ref = a.copy()
+        O.run("j2d5pt", ref, n, T)
+        assert np.array_equal(o[T % 2], ref[T % 2])

@@ -65,7 +65,7 @@ FIELDS = {
     "sweep_kernel_cap": ["bt", "ck", "cj", "nw", "exact", "waves_per_eu"],
     "plane_kernel": ["bt", "ck", "cj", "nw", "exact"],
     "direct_kernel": ["ck"],
-    "adjoint_kernel": ["bt", "ck", "ckpt"],
+    "adjoint_kernel": ["bt", "ck", "nw"],
 }
 
 

@@ -137,7 +137,7 @@ int default_sl(const Variant& v, int bt, const Geom& g, long long ntk, long long
   const long long n0 = g.D0 - 2 * g.r;
   const bool tiles3 = (v.kind == kSweep || v.kind == kPlane) && g.dim == 3;
   long long slots = 0;
-  if ((v.kind == kSweep || v.kind == kPlane) && bt >= 1 && bt <= kMaxBT && v.occ[bt]) {
+  if ((v.kind == kSweep || v.kind == kPlane || v.kind == kRows) && bt >= 1 && bt <= kMaxBT && v.occ[bt]) {
     const int occ = v.occ[bt](mode);
     const int cus = occ > 0 ? device_cus() : 0;
     slots = (long long)occ * cus * (tiles3 ? 1 : v.nw);  // workgroups (3D) or waves (2D)

@@ -110,7 +110,7 @@ double ops_per_update(const StencilRT& s, const Variant& v, int mode, bool has_d
   if (!s.linear) ops = 45.0;
   else if (mode == AN5D_FAST) ops = s.ntaps;
   else ops = 2.0 * s.ntaps - 1 + (has_div ? 6.0 : 0.0);
-  if (v.kind == kSweep) {
+  if (v.kind == kSweep || v.kind == kRows) {
     // DPP halo per row and the middle-dim exchange, per cell
     ops += 2.0 * s.rad / v.ck;
     if (s.dim == 3) ops += 2.0 * s.rad * 2.0 / v.cj;
@@ -136,7 +136,7 @@ double model_gstencil(const an5d_problem* p, const Variant& v, int bt, int sl, i
   const double n0 = g.D0 - 2.0 * g.r, nj = g.dim == 3 ? g.D1 - 2.0 * g.r : 1.0, nk = g.D2 - 2.0 * g.r;
   const double cells = n0 * nj * nk;
   double ovk = 1, ovj = 1, ovi = 1, waves;
-  if (v.kind == kSweep) {
+  if (v.kind == kSweep || v.kind == kRows) {
     ovk = lp.args.ntk * 64.0 * v.ck / nk;
     if (g.dim == 3) ovj = lp.args.ntj * (double)v.nw * v.cj / nj;
     ovi = (lp.args.nseg * (double)lp.args.sl + 2.0 * bt * g.r * lp.args.nseg) / n0;

@@ -31,7 +31,9 @@ inline hipError_t lds_opt_in(const void* kernel, size_t lds, std::atomic<int>* d
   return e;
 }
 
-enum KernelKind { kSweep = 0, kDirect = 1, kPlane = 2 };
+// kRows: one step per launch, rows streamed with per-row partial sums (2D
+// boxes); launched with a one-step sweep's layout
+enum KernelKind { kSweep = 0, kDirect = 1, kPlane = 2, kRows = 3 };
 enum DType { kF32 = 0, kF64 = 1 };
 
 // dtype-erased SweepArgs<T>

@@ -3,6 +3,7 @@
 // 42 translation units compile in parallel.
 #include "engine.hpp"
 #include "plane.hpp"
+#include "rows.hpp"
 #include "sweep.hpp"
 #include "variants.hpp"
 
@@ -132,6 +133,52 @@ int occupancy_plane(int mode) {
   return mode ? occupancy_plane_once<CK, CJ, NW, false>() : occupancy_plane_once<CK, CJ, NW, true>();
 }
 
+template <int CK, bool EXACT, int WPE>
+constexpr auto rows_fn() {
+  if constexpr (WPE > 0) return &rows_kernel_cap<SID, T, CK, 4, EXACT, WPE>;
+  else return &rows_kernel<SID, T, CK, 4, EXACT>;
+}
+
+template <int CK, bool EXACT, int WPE>
+hipError_t launch_rows(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  constexpr auto kern = rows_fn<CK, EXACT, WPE>();
+  hipLaunchKernelGGL(kern, grid, block, 0, st, to_args<T>(a));
+  return hipGetLastError();
+}
+
+template <int CK, bool EXACT, int WPE>
+int occupancy_rows_once() {
+  static const int n = [] {
+    int blocks = 0;
+    const void* k = reinterpret_cast<const void*>(rows_fn<CK, EXACT, WPE>());
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * 4, 0) != hipSuccess) return 0;
+    return blocks;
+  }();
+  return n;
+}
+template <int CK, int WPE>
+int occupancy_rows(int mode) {
+  return mode == 1 ? occupancy_rows_once<CK, false, WPE>() : occupancy_rows_once<CK, true, WPE>();
+}
+
+// Row-streamed partial-sum kernels for the 2D boxes of radius >= 2 (whose
+// register rings spill), one per run width CK, optionally register-capped.
+template <int CK, int WPE = 0>
+void add_rows(std::vector<Variant>& out) {
+  if constexpr (rows_order<Def<SID>>() && Def<SID>::rad >= 2 && CK >= Def<SID>::rad) {
+    Variant v{};
+    v.stencil = SID;
+    v.dtype = AN5D_F64 ? kF64 : kF32;
+    v.kind = kRows;
+    v.ck = CK; v.cj = 1; v.nw = 4; v.btmax = 1;
+    v.regcap = WPE > 0 ? 512 / WPE : 0;
+    v.fn[1] = &launch_rows<CK, true, WPE>;
+    v.fn_fast[1] = &launch_rows<CK, false, WPE>;
+    v.occ[1] = &occupancy_rows<CK, WPE>;
+    out.push_back(v);
+  }
+}
+
 // The shape's btmax lowered until the exchange buffers fit gfx950's 160 KB of LDS.
 template <int W>
 constexpr Shape fitted_shape() {
@@ -244,6 +291,10 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   add_capped<2>(out);
   add_capped<3>(out);
   add_capped<4>(out);
+  // row-streamed partial sums (2D boxes, radius >= 2)
+  add_rows<4>(out);
+  if constexpr (!AN5D_F64) add_rows<8>(out);
+  else add_rows<4, 2>(out);
 }
 
 }  // namespace an5d

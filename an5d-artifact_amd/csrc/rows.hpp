@@ -1,0 +1,199 @@
+// One step per launch for the 2D boxes, streamed by rows with per-row partial
+// sums (the reference's associative "partial-sum" code shape for box
+// stencils, re-designed for gfx950).
+//
+// The temporally blocked sweep keeps 2R+1 input rows per stage in registers,
+// each extended by R cells on both sides, which for radius >= 2 boxes (25 to
+// 81 taps) exceeds the register file and spills.  Here each input row is read
+// once and, extended by DPP, adds its 2R+1 row-convolutions into the partial
+// sums of the 2R+1 output rows it touches; an output row is complete (and
+// stored) after its last input row.  Registers: (2R+1) x CK partial sums and
+// one extended row, instead of (2R+1) extended rows per stage.
+//
+// Exactness: the sources list a box's taps row-major (d0, then d2, both
+// ascending: box2d4r.c:14-108), and the input rows arrive in ascending d0, so
+// every output accumulates the same rounded products in the same order as the
+// C expression (first term assigned, the others added; -ffp-contract=off):
+// the results are the C loop's bit for bit.  FMA mode chains fused
+// multiply-adds with the folded coefficients in the same order.
+//
+// Launch layout: that of a one-step sweep (wave tiles of 64*CK cells along
+// the row, R cells of overlap at each side; segments of sl rows), so the
+// host plans it like one.  Coefficients are re-read per row through a scalar
+// pointer the compiler cannot hoist (81 FP64 values do not fit the SGPRs;
+// scalar-cache loads per row cost less than holding them in VGPRs).
+#pragma once
+
+#include "engine.hpp"
+#include "sweep.hpp"
+
+namespace an5d {
+
+// Taps ordered row-major (d0 then d2 ascending) over the full (2R+1)^2 box:
+// the condition for the partial-sum form to keep the source order.
+template <class D>
+constexpr bool rows_order() {
+  if (D::dim != 2 || !D::linear) return false;
+  constexpr int R = D::rad, P = 2 * R + 1;
+  if (D::ntaps != P * P) return false;
+  for (int q = 0; q < D::ntaps; q++)
+    if (D::taps[q].d0 != q / P - R || D::taps[q].d2 != q % P - R || D::taps[q].d1 != 0) return false;
+  return true;
+}
+
+template <int ID, typename T, int CK, int NW, bool EXACT, int DM>
+__device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
+#pragma clang fp contract(off)
+  using D = Def<ID>;
+  constexpr int R = D::rad;
+  constexpr int P = 2 * R + 1;
+  constexpr int NT = D::ntaps;
+  constexpr unsigned SZ = sizeof(T);
+  static_assert(rows_order<D>(), "row-major box taps only");
+  static_assert(CK >= R, "a lane must hold at least R cells");
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = blockIdx.x * NW + w;
+  // the coefficients in LDS, read one box row at a time (broadcast reads);
+  // filled before any wave of the workgroup leaves
+  __shared__ T cs[NT];
+  for (int q = threadIdx.x; q < NT; q += 64 * NW) cs[q] = a.coef[q];
+  __syncthreads();
+  if (gw >= a.ntk * a.nseg) return;
+  const int tk = gw % a.ntk, seg = gw / a.ntk;
+  const int kv0 = R + tk * a.SK;
+  const int kv1 = min(kv0 + a.SK, a.D2 - R);
+  const int kl = kv0 - R + lane * CK;
+  const int i0 = R + seg * a.sl;
+  const int sle = min(a.sl, a.D0 - R - i0);
+  const int nit = sle + 2 * R;
+
+  bool kst[CK];
+  bool kst_all = true;
+  unsigned vclamp[CK];
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) {
+    int k = kl + kk;
+    kst[kk] = k >= kv0 && k < kv1;
+    kst_all = kst_all && kst[kk];
+    k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+    vclamp[kk] = (unsigned)k * SZ;
+  }
+  const T div = a.coef[NT];
+  const T rc = a.coef[NT + 1];
+
+  // acc[d] holds output m = n - d (input row n = the one being added): box
+  // row d of it arrives now; acc[P-1] completes with this row
+  T acc[P][CK];
+#pragma unroll
+  for (int s = 0; s < P; s++)
+#pragma unroll
+    for (int kk = 0; kk < CK; kk++) acc[s][kk] = T(0);
+
+  auto load_row = [&](T (&x)[CK], int n) {
+    int r = i0 - R + n;
+    r = r < 0 ? 0 : (r >= a.D0 ? a.D0 - 1 : r);
+    const auto rs = make_rsrc(a.src + (long long)r * a.s0);
+#pragma unroll
+    for (int kk = 0; kk < CK; kk++) x[kk] = bload<false, T>(rs, vclamp[kk], 0);
+  };
+  T nxt[CK];
+  load_row(nxt, 0);
+  for (int n = 0; n < nit; n++) {
+    T x[CK];
+#pragma unroll
+    for (int kk = 0; kk < CK; kk++) x[kk] = nxt[kk];
+    load_row(nxt, n + 1);  // one row ahead (the last is a clamped stand-in)
+    T xe[CK + 2 * R];
+#pragma unroll
+    for (int kk = 0; kk < CK; kk++) xe[kk + R] = x[kk];
+#pragma unroll
+    for (int m = 1; m <= R; m++) {
+      xe[R - m] = from_left(x[CK - m]);
+      xe[R + CK - 1 + m] = from_right(x[m - 1]);
+    }
+    static_for<0, P>([&](auto dc) {
+      constexpr int DI = decltype(dc)::value;
+      // this box row's coefficients, read through an offset the compiler
+      // cannot hoist out of the loop (81 FP64 values would not fit), after a
+      // scheduling barrier that keeps the box rows' reads apart
+      int off = DI * P;
+      asm volatile("" : "+s"(off));
+      // nothing of the next box row is scheduled before this one's sums
+      __builtin_amdgcn_sched_barrier(0);
+      T cr[P];
+#pragma unroll
+      for (int d = 0; d < P; d++) cr[d] = cs[off + d];
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) {
+        T s = acc[DI][kk];
+        static_for<0, P>([&](auto ec) {
+          constexpr int E = decltype(ec)::value;
+          constexpr TapDef t = D::taps[DI * P + E];
+          const T v = xe[kk + E];
+          if constexpr (EXACT) {
+            const T prod = cr[E] * v;
+            if constexpr (DI == 0 && E == 0) s = t.sign > 0 ? prod : -prod;
+            else if constexpr (t.sign > 0) s = s + prod;
+            else s = s - prod;
+          } else {
+            if constexpr (DI == 0 && E == 0) s = cr[E] * v;
+            else s = fma_t(cr[E], v, s);
+          }
+        });
+        acc[DI][kk] = s;
+      }
+    });
+    // output m = n - 2R received its last row
+    const int m = n - 2 * R;
+    if (m >= 0) {
+      T out[CK];
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) {
+        if constexpr (DM > 0) out[kk] = div_by(acc[P - 1][kk], div, rc, DM);
+        else out[kk] = acc[P - 1][kk];
+      }
+      T* row = a.dst + (long long)(i0 + m) * a.s0 + kl;
+      if (kst_all) {
+        if (a.nt_store) gstore_run<true, T, CK>(row, out);
+        else gstore_run<false, T, CK>(row, out);
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++)
+          if (kst[kk]) row[kk] = out[kk];
+      }
+    }
+#pragma unroll
+    for (int d = P - 1; d > 0; d--)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) acc[d][kk] = acc[d - 1][kk];
+  }
+}
+
+template <int ID, typename T, int CK, int NW, bool EXACT>
+__device__ __forceinline__ void rows_dispatch(const SweepArgs<T>& a) {
+  if constexpr (!EXACT) {
+    rows_body<ID, T, CK, NW, false, 0>(a);
+  } else {
+    if (a.has_div == 2) rows_body<ID, T, CK, NW, true, 2>(a);
+    else if (a.has_div == 1) rows_body<ID, T, CK, NW, true, 1>(a);
+    else rows_body<ID, T, CK, NW, true, 0>(a);
+  }
+}
+
+template <int ID, typename T, int CK, int NW, bool EXACT>
+__global__ __launch_bounds__(64 * NW) void rows_kernel(SweepArgs<T> a) {
+  rows_dispatch<ID, T, CK, NW, EXACT>(a);
+}
+
+// Register-capped twin (at least WPE waves per SIMD): one wave per SIMD
+// leaves the FP64 boxes' dependent chains and row loads exposed; the cap
+// trades spills for a second wave.  The tuner measures both.
+template <int ID, typename T, int CK, int NW, bool EXACT, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void rows_kernel_cap(
+    SweepArgs<T> a) {
+  rows_dispatch<ID, T, CK, NW, EXACT>(a);
+}
+
+}  // namespace an5d

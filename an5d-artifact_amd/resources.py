@@ -66,6 +66,8 @@ FIELDS = {
     "plane_kernel": ["bt", "ck", "cj", "nw", "exact"],
     "direct_kernel": ["ck"],
     "adjoint_kernel": ["bt", "ck", "nw"],
+    "rows_kernel": ["ck", "nw", "exact"],
+    "rows_kernel_cap": ["ck", "nw", "exact", "waves_per_eu"],
 }
 
 

@@ -233,6 +233,19 @@ extern "C" int an5d_adjoint_workspace(const an5d_problem* p, const an5d_config* 
   int rc = plan_adjoint(p, cfg, ap);
   if (rc) return rc;
   if (bytes) *bytes = ap.ws_bytes;
+  // one padded grid per bt-step block: the footprint grows with T / bt.  A
+  // workspace larger than the current device's memory can never be
+  // allocated: say so here rather than as a failed allocation later (no
+  // check without a device).
+  size_t free_b = 0, total_b = 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0 &&
+      ap.ws_bytes > total_b)
+    return fail(AN5D_E_ALLOC, "adjoint workspace of " + std::to_string(ap.ws_bytes >> 20) + " MiB (" +
+                                  std::to_string(ap.blocks.size() + 3) +
+                                  " padded grids: one per bt-step block) exceeds the device's " +
+                                  std::to_string(total_b >> 20) + " MiB; use a larger bt or fewer steps");
+  (void)hipGetLastError();
   return AN5D_OK;
 }
 

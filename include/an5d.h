@@ -170,7 +170,10 @@ int an5d_init_grid(int dtype, void* A, long long cells, long long seed_base, int
  * every bt steps; each reverse launch recomputes the
  * states of its block from the checkpoint and runs the transposed sweep for
  * bt steps.  The gradient and loss sums are reduced in a fixed order, so
- * repeated calls give identical results. */
+ * repeated calls give identical results.  The workspace holds one padded
+ * grid per bt-step block plus three (T/bt + 3 grids: 8192^2 FP32 at T=1000,
+ * bt=4 is about 75 GB); a size beyond the current device's memory is
+ * rejected with AN5D_E_ALLOC and a message. */
 int an5d_adjoint_workspace(const an5d_problem* prob, const an5d_config* cfg, size_t* bytes);
 /* Device form: enqueues on `stream`; *ms_fwd (optional) gets the time of the
  * checkpointed forward pass (synchronises on an event when non-NULL).

@@ -194,3 +194,12 @@ def test_autograd_matches_oracle_fp32(torch_cuda):
     assert abs(loss.item() - ol) <= 1e-4 * ol
     assert np.max(np.abs(tu.grad.cpu().numpy() - og)) <= 2e-4 * np.max(np.abs(og))
     assert np.max(np.abs(tc.grad.cpu().numpy() - ogc)) <= 2e-4 * np.max(np.abs(ogc))
+
+
+def test_adjoint_workspace_beyond_device_memory_is_rejected(torch_cuda):
+    """One padded grid per bt-step block: 16384^2 FP32 over 20000 steps at
+    bt=1 needs about 22 TB, which the library refuses with a message before
+    anything is allocated."""
+    print("case adjoint workspace too large", flush=True)
+    with pytest.raises(pyan5d.An5dError, match="exceeds the device"):
+        pyan5d.AdjointWorkspace("j2d5pt", [16384, 16384], 20000, np.float32, bt=1)

@@ -69,8 +69,8 @@ struct AdjPlan {
 static int adj_launch_plan(const AdjPlan& ap, int k, AdjLaunch& l, dim3& grid, dim3& block,
                            bool sizing = false);
 
-// The forward pass runs at its own temporal block btf (the multiple of bt
-// nearest the tuned forward depth) and stores the state at every multiple of
+// The forward pass runs at its own temporal block btf (the largest multiple
+// of bt not above the tuned forward depth) and stores the state at every multiple of
 // bt: a launch's output is the checkpoint where it ends, and the intermediate
 // stages that reach a block boundary inside it store theirs.
 static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& ap) {
@@ -95,9 +95,12 @@ static int plan_adjoint(const an5d_problem* p, const an5d_config* cfg, AdjPlan& 
   if (ap.fwd->kind == kDirect) bt = 1;
   ap.bt = bt;
   // every launch ends on a block boundary (bt <= btmax, so m = 1 always fits);
-  // a tie goes to the smaller m: a launch that stores no intermediate state is
-  // cheaper per step than one that does at the same distance from bt_pref
-  int m = (bt_pref + (bt - 1) / 2) / bt;
+  // the largest multiple of bt not above bt_pref: a deeper launch that also
+  // stores intermediate states holds more registers than the tuned depth
+  // (measured at 8192^2, T=256: j2d9pt bt=3 14.7 ms at m=1 against 15.3 at
+  // m=2; j2d9pt-gol bt=4 11.0 against 12.0; at bt=3 / bt=2, where the
+  // multiple stays within the tuned depth, m=2 won)
+  int m = bt_pref / bt;
   if (m < 1) m = 1;
   while (m > 1 && (m * bt > ap.fwd->btmax || m * bt > kMaxMidBT)) m--;
   int btf = m * bt;

@@ -19,9 +19,9 @@
 //
 // Launch layout: that of a one-step sweep (wave tiles of 64*CK cells along
 // the row, R cells of overlap at each side; segments of sl rows), so the
-// host plans it like one.  Coefficients are re-read per row through a scalar
-// pointer the compiler cannot hoist (81 FP64 values do not fit the SGPRs;
-// scalar-cache loads per row cost less than holding them in VGPRs).
+// host plans it like one.  The coefficients live in LDS and are read one box
+// row at a time (81 FP64 values would not fit the registers next to the
+// partial sums).
 #pragma once
 
 #include "engine.hpp"

@@ -113,24 +113,29 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
       xe[R - m] = from_left(x[CK - m]);
       xe[R + CK - 1 + m] = from_right(x[m - 1]);
     }
-    static_for<0, P>([&](auto dc) {
-      constexpr int DI = decltype(dc)::value;
-      // this box row's coefficients, read through an offset the compiler
-      // cannot hoist out of the loop (81 FP64 values would not fit), after a
-      // scheduling barrier that keeps the box rows' reads apart
-      int off = DI * P;
+    // box row DI's coefficients are in crb[DI & 1]; row DI + 1's are read
+    // while row DI is summed.  Their offsets pass through an empty asm, so
+    // the reads stay in the loop (81 FP64 values would not fit next to the
+    // partial sums) and none is issued before the previous row's sums.
+    T crb[2][P];
+    auto read_coef = [&](T (&cr)[P], int di) {
+      int off = di * P;
       asm volatile("" : "+s"(off));
-      // nothing of the next box row is scheduled before this one's sums
-      __builtin_amdgcn_sched_barrier(0);
-      T cr[P];
 #pragma unroll
       for (int d = 0; d < P; d++) cr[d] = cs[off + d];
+    };
+    read_coef(crb[0], 0);
+    static_for<0, P>([&](auto dc) {
+      constexpr int DI = decltype(dc)::value;
+      if constexpr (DI + 1 < P) read_coef(crb[(DI + 1) & 1], DI + 1);
+      const T(&cr)[P] = crb[DI & 1];
+      // tap by tap over the CK cells: each cell's sum keeps the source order
+      static_for<0, P>([&](auto ec) {
+        constexpr int E = decltype(ec)::value;
+        constexpr TapDef t = D::taps[DI * P + E];
 #pragma unroll
-      for (int kk = 0; kk < CK; kk++) {
-        T s = acc[DI][kk];
-        static_for<0, P>([&](auto ec) {
-          constexpr int E = decltype(ec)::value;
-          constexpr TapDef t = D::taps[DI * P + E];
+        for (int kk = 0; kk < CK; kk++) {
+          T& s = acc[DI][kk];
           const T v = xe[kk + E];
           if constexpr (EXACT) {
             const T prod = cr[E] * v;
@@ -141,9 +146,15 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
             if constexpr (DI == 0 && E == 0) s = cr[E] * v;
             else s = fma_t(cr[E], v, s);
           }
-        });
-        acc[DI][kk] = s;
-      }
+        }
+      });
+      // the sums of this box row are formed here: without this the compiler
+      // sinks them below the store branch, and the reads of every box row's
+      // coefficients then sit together at the top of the loop (81 values
+      // live at once: for box2d4r FP64, 284 registers and one wave per SIMD
+      // instead of about 150 and three)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) asm volatile("" : "+v"(acc[DI][kk]));
     });
     // output m = n - 2R received its last row
     const int m = n - 2 * R;
@@ -187,9 +198,10 @@ __global__ __launch_bounds__(64 * NW) void rows_kernel(SweepArgs<T> a) {
   rows_dispatch<ID, T, CK, NW, EXACT>(a);
 }
 
-// Register-capped twin (at least WPE waves per SIMD): one wave per SIMD
-// leaves the FP64 boxes' dependent chains and row loads exposed; the cap
-// trades spills for a second wave.  The tuner measures both.
+// Register-capped twin (at least WPE waves per SIMD): the bound lets the
+// register allocator aim for more waves than the workgroup size alone asks
+// for (box2d4r FP64 exact: 127 VGPRs and four waves per SIMD, against 156 and
+// three).  The tuner measures both.
 template <int ID, typename T, int CK, int NW, bool EXACT, int WPE>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void rows_kernel_cap(
     SweepArgs<T> a) {

@@ -179,6 +179,45 @@ void add_rows(std::vector<Variant>& out) {
   }
 }
 
+template <int CK, int CJ, int NW, bool EXACT>
+hipError_t launch_rows3(const LaunchArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((rows3_kernel<SID, T, CK, CJ, NW, EXACT>), grid, block, 0, st, to_args<T>(a));
+  return hipGetLastError();
+}
+
+template <int CK, int CJ, int NW, bool EXACT>
+int occupancy_rows3_once() {
+  static const int n = [] {
+    int blocks = 0;
+    const void* k = reinterpret_cast<const void*>(&rows3_kernel<SID, T, CK, CJ, NW, EXACT>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64 * NW, 0) != hipSuccess) return 0;
+    return blocks;
+  }();
+  return n;
+}
+template <int CK, int CJ, int NW>
+int occupancy_rows3(int mode) {
+  return mode == 1 ? occupancy_rows3_once<CK, CJ, NW, false>() : occupancy_rows3_once<CK, CJ, NW, true>();
+}
+
+// Plane-streamed partial sums for the radius-1 3D boxes (taps grouped by
+// plane): one step per launch on the plane kernel's whole tiles (nothing
+// recomputed), so they are planned like it (kind kPlane).
+template <int CK, int CJ, int NW>
+void add_rows3(std::vector<Variant>& out) {
+  if constexpr (rows3_order<Def<SID>>() && is_box<Def<SID>>() && Def<SID>::rad == 1) {
+    Variant v{};
+    v.stencil = SID;
+    v.dtype = AN5D_F64 ? kF64 : kF32;
+    v.kind = kPlane;
+    v.ck = CK; v.cj = CJ; v.nw = NW; v.btmax = 1;
+    v.fn[1] = &launch_rows3<CK, CJ, NW, true>;
+    v.fn_fast[1] = &launch_rows3<CK, CJ, NW, false>;
+    v.occ[1] = &occupancy_rows3<CK, CJ, NW>;
+    out.push_back(v);
+  }
+}
+
 // The shape's btmax lowered until the exchange buffers fit gfx950's 160 KB of LDS.
 template <int W>
 constexpr Shape fitted_shape() {
@@ -295,6 +334,10 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   add_rows<4>(out);
   if constexpr (!AN5D_F64) add_rows<8>(out);
   else add_rows<4, 2>(out);
+  // plane-streamed partial sums (3D radius-1 boxes)
+  add_rows3<2, 4, 4>(out);
+  if constexpr (!AN5D_F64) add_rows3<4, 2, 8>(out);
+  else add_rows3<2, 2, 8>(out);
 }
 
 }  // namespace an5d

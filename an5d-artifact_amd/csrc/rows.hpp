@@ -208,4 +208,227 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
   rows_dispatch<ID, T, CK, NW, EXACT>(a);
 }
 
+
+// ---- 3D: plane-streamed partial sums for the radius-1 boxes -----------------
+//
+// The register sweep of a 27-point box keeps three extended planes per stage
+// (237 VGPRs for j3d27pt FP64 at one step: two waves per SIMD) and
+// recomputes the overlap of its tiles.  Here, one step per launch, each
+// input plane is read once and adds its in-plane taps into the partial sums
+// of the 2R+1 output planes it touches, as the 2D kernel above does with
+// rows:
+//   - a wave owns CJ rows x 64*CK cells of every plane of its segment
+//     (workgroups of NW waves stacked along dim 1, the plane kernel's tiles:
+//     nothing is recomputed); it reads its rows plus R rows above and below
+//     straight from memory (neighbouring waves' rows, mostly from L2), so the
+//     waves need no LDS exchange and no barrier;
+//   - cells of the lanes' neighbours come by DPP; lanes 0 and 63 read their
+//     R cells beyond the tile edge themselves;
+//   - partial sums: (2R+1) x CJ x CK per lane; a plane's coefficients are
+//     read from LDS one dim-0 offset at a time.
+// Exactness: the taps must be grouped by plane in ascending d0 (j3d27pt.c
+// and box3d1r.c list the d0 = -1 terms, then 0, then +1); within a plane the
+// source order is kept, so each output adds the same rounded products in the
+// same order as the C expression.
+template <class D>
+constexpr bool rows3_order() {
+  if (D::dim != 3 || !D::linear) return false;
+  for (int q = 1; q < D::ntaps; q++)
+    if (D::taps[q].d0 < D::taps[q - 1].d0) return false;
+  return true;
+}
+
+// first tap with d0 >= v (taps grouped by plane)
+template <class D>
+constexpr int first_tap_from(int v) {
+  for (int q = 0; q < D::ntaps; q++)
+    if (D::taps[q].d0 >= v) return q;
+  return D::ntaps;
+}
+
+template <int ID, typename T, int CK, int CJ, int NW, bool EXACT, int DM>
+__device__ __forceinline__ void rows3_body(const SweepArgs<T>& a) {
+#pragma clang fp contract(off)
+  using D = Def<ID>;
+  constexpr int R = D::rad;
+  constexpr int P = 2 * R + 1;
+  constexpr int NT = D::ntaps;
+  constexpr unsigned SZ = sizeof(T);
+  constexpr int TK = 64 * CK, TJ = NW * CJ;
+  constexpr int LR = CJ + 2 * R;  // rows a wave reads per plane
+  static_assert(rows3_order<D>(), "taps grouped by plane in ascending d0 only");
+  static_assert(CK >= R, "a lane must hold at least R cells");
+
+  __shared__ T cs[NT];
+  for (int q = threadIdx.x; q < NT; q += 64 * NW) cs[q] = a.coef[q];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int tk = b % a.ntk;
+  b /= a.ntk;
+  const int tj = b % a.ntj;
+  const int seg = b / a.ntj;
+  const int k0 = R + tk * TK;           // first cell of the tile
+  const int j0 = R + tj * TJ + w * CJ;  // first own row of the wave
+  const int i0 = R + seg * a.sl;
+  const int sle = min(a.sl, a.D0 - R - i0);
+  if (sle <= 0) return;  // the whole workgroup: uniform
+  const int nit = sle + 2 * R;
+  const int kl = k0 + lane * CK;
+
+  bool kst[CK];
+  bool kst_all = true;
+  unsigned vk[CK];
+#pragma unroll
+  for (int kk = 0; kk < CK; kk++) {
+    int k = kl + kk;
+    kst[kk] = k < a.D2 - R;
+    kst_all = kst_all && kst[kk];
+    k = k >= a.D2 ? a.D2 - 1 : k;
+    vk[kk] = (unsigned)k * SZ;
+  }
+  // beyond the tile edge: lane 0 the R cells left of k0, lane 63 the R cells
+  // right of k0 + TK; the other lanes read their own first cell (unused)
+  unsigned ve[R];
+#pragma unroll
+  for (int m = 0; m < R; m++) {
+    int k = lane == 0 ? k0 - R + m : (lane == 63 ? k0 + TK + m : kl);
+    k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+    ve[m] = (unsigned)k * SZ;
+  }
+  const T div = a.coef[NT];
+  const T rc = a.coef[NT + 1];
+
+  // acc[d] holds output plane m = n - d while input plane n is added
+  T acc[P][CJ][CK];
+#pragma unroll
+  for (int d = 0; d < P; d++)
+#pragma unroll
+    for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) acc[d][jj][kk] = T(0);
+
+  struct Plane {
+    T x[LR][CK];
+    T e[LR][R];
+  };
+  auto load_plane = [&](Plane& pl, int n) {
+    int p = i0 - R + n;
+    p = p < 0 ? 0 : (p >= a.D0 ? a.D0 - 1 : p);
+    const T* base = a.src + (long long)p * a.s0;
+#pragma unroll
+    for (int r = 0; r < LR; r++) {
+      int j = j0 - R + r;
+      j = j < 0 ? 0 : (j >= a.D1 ? a.D1 - 1 : j);
+      const auto rs = make_rsrc(base + (long long)j * a.s1);
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) pl.x[r][kk] = bload<false, T>(rs, vk[kk], 0);
+#pragma unroll
+      for (int m = 0; m < R; m++) pl.e[r][m] = bload<false, T>(rs, ve[m], 0);
+    }
+  };
+
+  Plane nxt;
+  load_plane(nxt, 0);
+  for (int n = 0; n < nit; n++) {
+    Plane cur = nxt;
+    load_plane(nxt, n + 1);  // one plane ahead (the last is a clamped stand-in)
+    // extended rows: xe[r][c] is cell kl - R + c of row j0 - R + r
+    T xe[LR][CK + 2 * R];
+#pragma unroll
+    for (int r = 0; r < LR; r++) {
+#pragma unroll
+      for (int kk = 0; kk < CK; kk++) xe[r][kk + R] = cur.x[r][kk];
+#pragma unroll
+      for (int m = 1; m <= R; m++) {
+        const T l = from_left(cur.x[r][CK - m]);
+        const T rr = from_right(cur.x[r][m - 1]);
+        xe[r][R - m] = lane == 0 ? cur.e[r][R - m] : l;
+        xe[r][R + CK - 1 + m] = lane == 63 ? cur.e[r][m - 1] : rr;
+      }
+    }
+    static_for<0, P>([&](auto dc) {
+      constexpr int DI = decltype(dc)::value;  // this plane is d0 = DI - R of output n - DI
+      constexpr int QA = first_tap_from<D>(DI - R), QB = first_tap_from<D>(DI - R + 1);
+      constexpr int NQ = QB - QA;
+      if constexpr (NQ > 0) {
+        // the block's coefficients through an offset the compiler cannot
+        // hoist out of the loop
+        int off = QA;
+        asm volatile("" : "+s"(off));
+        T cr[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) cr[q] = cs[off + q];
+        static_for<0, NQ>([&](auto qc) {
+          constexpr int QI = decltype(qc)::value;
+          constexpr TapDef t = D::taps[QA + QI];
+#pragma unroll
+          for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+            for (int kk = 0; kk < CK; kk++) {
+              T& s = acc[DI][jj][kk];
+              const T v = xe[jj + R + t.d1][kk + R + t.d2];
+              if constexpr (EXACT) {
+                const T prod = cr[QI] * v;
+                if constexpr (QA + QI == 0) s = t.sign > 0 ? prod : -prod;
+                else if constexpr (t.sign > 0) s = s + prod;
+                else s = s - prod;
+              } else {
+                if constexpr (QA + QI == 0) s = cr[QI] * v;
+                else s = fma_t(cr[QI], v, s);
+              }
+            }
+        });
+        // formed here, not sunk below the store branch
+#pragma unroll
+        for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) asm volatile("" : "+v"(acc[DI][jj][kk]));
+      }
+    });
+    // output plane m = n - 2R received its last input plane
+    const int m = n - 2 * R;
+    if (m >= 0) {
+#pragma unroll
+      for (int jj = 0; jj < CJ; jj++) {
+        if (j0 + jj >= a.D1 - R) continue;  // uniform
+        T out[CK];
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) {
+          if constexpr (DM > 0) out[kk] = div_by(acc[P - 1][jj][kk], div, rc, DM);
+          else out[kk] = acc[P - 1][jj][kk];
+        }
+        T* row = a.dst + (long long)(i0 + m) * a.s0 + (long long)(j0 + jj) * a.s1 + kl;
+        if (kst_all) {
+          if (a.nt_store) gstore_run<true, T, CK>(row, out);
+          else gstore_run<false, T, CK>(row, out);
+        } else {
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++)
+            if (kst[kk]) row[kk] = out[kk];
+        }
+      }
+    }
+#pragma unroll
+    for (int d = P - 1; d > 0; d--)
+#pragma unroll
+      for (int jj = 0; jj < CJ; jj++)
+#pragma unroll
+        for (int kk = 0; kk < CK; kk++) acc[d][jj][kk] = acc[d - 1][jj][kk];
+  }
+}
+
+template <int ID, typename T, int CK, int CJ, int NW, bool EXACT>
+__global__ __launch_bounds__(64 * NW) void rows3_kernel(SweepArgs<T> a) {
+  if constexpr (!EXACT) {
+    rows3_body<ID, T, CK, CJ, NW, false, 0>(a);
+  } else {
+    if (a.has_div == 2) rows3_body<ID, T, CK, CJ, NW, true, 2>(a);
+    else if (a.has_div == 1) rows3_body<ID, T, CK, CJ, NW, true, 1>(a);
+    else rows3_body<ID, T, CK, CJ, NW, true, 0>(a);
+  }
+}
+
 }  // namespace an5d

@@ -68,6 +68,7 @@ FIELDS = {
     "adjoint_kernel": ["bt", "ck", "nw"],
     "rows_kernel": ["ck", "nw", "exact"],
     "rows_kernel_cap": ["ck", "nw", "exact", "waves_per_eu"],
+    "rows3_kernel": ["ck", "cj", "nw", "exact"],
 }
 
 

@@ -85,8 +85,11 @@ typedef struct an5d_stencil_info {
 
 typedef struct an5d_variant_info {
   int kind;  /* 0: temporally blocked sweep, 1: one step per launch from global
-               memory, 2: one step per launch with the plane staged in LDS
-               (3D stars of radius 2-4) */
+               memory, 2: one step per launch on whole tiles, nothing
+               recomputed (3D stars of radius 2-4: the plane staged in LDS;
+               3D radius-1 boxes: plane-streamed partial sums), 3: one step
+               per launch, row-streamed partial sums (2D boxes of radius
+               2-4) */
   int ck, cj, nw, btmax;
   long long lds_bytes;
   int regcap; /* registers per lane the compiler was held to in the deeper

@@ -60,10 +60,13 @@ def test_every_stencil_has_kernels_for_both_dtypes(name):
         assert vs, name
         kinds = [v["kind"] for v in vs]
         assert kinds.count(1) == 1  # the one-step-per-launch kernel is always there
-        # the LDS-staged one-step kernel exactly for the radius 2-4 3D stars
+        # one-step kernels on whole tiles: the LDS-staged plane kernel exactly
+        # for the radius 2-4 3D stars, two plane-streamed partial-sum shapes
+        # exactly for the radius-1 3D boxes
         d = O.info(name)
         is_star3 = name.startswith("star3d") and d["rad"] >= 2
-        assert kinds.count(2) == (1 if is_star3 else 0), (name, kinds)
+        is_box3_r1 = name in ("j3d27pt", "box3d1r")
+        assert kinds.count(2) == (1 if is_star3 else 2 if is_box3_r1 else 0), (name, kinds)
         for v in vs:
             assert 1 <= v["btmax"] <= 16
             if v["kind"] == 2:

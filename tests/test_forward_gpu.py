@@ -201,6 +201,48 @@ def test_plane_kernel_ragged_grids(name, dt):
         assert rel <= (1e-5 if dt == np.float32 else 1e-12), rel
 
 
+@pytest.mark.parametrize("name", ["j3d27pt", "box3d1r"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_box3d_plane_partial_sums_ragged_grids(name, dt):
+    """The plane-streamed partial-sum kernels of the radius-1 3D boxes (kind 2
+    for a box): bit for bit the oracle on grids smaller than one tile, ragged
+    tiles in both in-plane dims, segments of 1-7 planes and the default, in
+    exact mode; within 1e-5 / 1e-12 relative in FMA mode; j3d27pt also with
+    its /159 divisor and other coefficients."""
+    vs = [v for v in pyan5d.variants(name, 0 if dt == np.float32 else 1) if v["kind"] == 2]
+    assert len(vs) == 2
+    for v in vs:
+        for n, sl in [([1, 1, 1], 0), ([3, 17, 130], 1), ([9, 5, 300], 2), ([20, 33, 257], 7),
+                      ([13, 16, 128], 0), ([40, 70, 260], 0)]:
+            T = 4
+            print(f"case box3d partial sums {name} {np.dtype(dt).name} variant {v['index']} n={n} sl={sl}",
+                  flush=True)
+            A = rand_grid(name, n, dt, seed=sum(n) + v["index"])
+            B = A.copy()
+            pyan5d.forward_host(name, A, n, T, variant=v["index"], sl=sl)
+            O.run(name, B, n, T)
+            assert same(A[T % 2], B[T % 2])
+            C = rand_grid(name, n, dt, seed=sum(n) + 1)
+            D = C.copy()
+            pyan5d.forward_host(name, C, n, T, variant=v["index"], sl=sl, mode=pyan5d.FAST)
+            O.run(name, D, n, T)
+            got, ref = C[T % 2].astype(np.float64), D[T % 2].astype(np.float64)
+            rel = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), np.finfo(dt).tiny))
+            assert rel <= (1e-5 if dt == np.float32 else 1e-12), rel
+        if name == "j3d27pt":
+            n, T = [30, 40, 200], 5
+            rng = np.random.default_rng(3)
+            coefs = rng.random(27).astype(np.float64)
+            for div in (159.0, 0.0):
+                print(f"case box3d partial sums {name} {np.dtype(dt).name} variant {v['index']} "
+                      f"divisor {div}", flush=True)
+                A = rand_grid(name, n, dt, seed=9)
+                B = A.copy()
+                pyan5d.forward_host(name, A, n, T, coefs=coefs, divisor=div, variant=v["index"])
+                O.run(name, B, n, T, coefs=coefs, divisor=div)
+                assert same(A[T % 2], B[T % 2])
+
+
 @pytest.mark.parametrize("name", ["j2d5pt", "j2d9pt", "star2d3r", "box2d2r", "gradient2d",
                                   "star3d1r", "box3d1r", "j3d27pt", "star3d4r"])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])

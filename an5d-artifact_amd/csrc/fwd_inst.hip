@@ -336,7 +336,7 @@ void AN5D_CAT(register_variants_, AN5D_SID, AN5D_F64)(std::vector<Variant>& out)
   else add_rows<4, 2>(out);
   // plane-streamed partial sums (3D radius-1 boxes)
   add_rows3<2, 4, 4>(out);
-  if constexpr (!AN5D_F64) add_rows3<4, 2, 8>(out);
+  if constexpr (!AN5D_F64) add_rows3<1, 8, 4>(out);
   else add_rows3<2, 2, 8>(out);
 }
 

@@ -1,7 +1,8 @@
 """Parity at the benchmark sizes: star3d1r 512^3 (FP32, FP64), j2d5pt FP32
-16384^2 and star3d4r 512^3 (FP32, FP64; the LDS-staged one-step kernel), each
-with the configuration the benchmark runs (explicit flags, else the
-autotuner's table, else the defaults).
+16384^2, star3d4r 512^3 (FP32, FP64; the LDS-staged one-step kernel),
+box2d4r 16384^2 (FP32, FP64; the row partial sums) and j3d27pt 512^3 (FP32,
+FP64; the plane partial sums), each with the configuration the benchmark
+runs (explicit flags, else the autotuner's table, else the defaults).
 
 - The first launches are compared with the oracle bit for bit (the oracle
   finishes a few steps of these grids in seconds).
@@ -30,7 +31,11 @@ FULL = [("star3d1r", np.float32, [512, 512, 512]),
         ("star3d1r", np.float64, [512, 512, 512]),
         ("j2d5pt", np.float32, [16384, 16384]),
         ("star3d4r", np.float32, [512, 512, 512]),
-        ("star3d4r", np.float64, [512, 512, 512])]
+        ("star3d4r", np.float64, [512, 512, 512]),
+        ("box2d4r", np.float32, [16384, 16384]),
+        ("box2d4r", np.float64, [16384, 16384]),
+        ("j3d27pt", np.float32, [512, 512, 512]),
+        ("j3d27pt", np.float64, [512, 512, 512])]
 
 
 def same(a, b):

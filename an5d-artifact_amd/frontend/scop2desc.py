@@ -31,8 +31,9 @@ import re
 import sys
 from dataclasses import dataclass, field
 
-# Stencil set evaluated by the artifact (SURVEY inventory; 3d125pt.c excluded:
-# it is in no run script and has no generated code).
+# Stencil set evaluated by the artifact (compiled/run.sh and the generated
+# code under compiled/; 3d125pt.c excluded: it is in no run script and has no
+# generated code).
 STENCILS = [
     "star2d1r", "star2d2r", "star2d3r", "star2d4r",
     "box2d1r", "box2d2r", "box2d3r", "box2d4r",

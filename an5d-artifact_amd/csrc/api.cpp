@@ -199,7 +199,8 @@ bool plan_launch(const Variant& v, int bt, const Geom& g, int sl, LaunchPlan& lp
     err = "plane of 2 GiB or more: the register-streamed kernels address planes with 32-bit offsets";
     return false;
   }
-  const long long SK = 64LL * v.ck - 2LL * bt * R;
+  // the row partial-sum kernels read their tile edges themselves: no overlap
+  const long long SK = v.kind == kRows ? 64LL * v.ck : 64LL * v.ck - 2LL * bt * R;
   if (SK <= 0) { err = "bt too deep for this variant's tile width"; return false; }
   const long long ntk = cdiv(nk, SK);
   long long SJ = 1, ntj = 1;

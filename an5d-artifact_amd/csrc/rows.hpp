@@ -17,11 +17,11 @@
 // the results are the C loop's bit for bit.  FMA mode chains fused
 // multiply-adds with the folded coefficients in the same order.
 //
-// Launch layout: that of a one-step sweep (wave tiles of 64*CK cells along
-// the row, R cells of overlap at each side; segments of sl rows), so the
-// host plans it like one.  The coefficients live in LDS and are read one box
-// row at a time (81 FP64 values would not fit the registers next to the
-// partial sums).
+// Launch layout: wave tiles of 64*CK cells along the row that do not overlap
+// (lanes 0 and 63 read the R cells beyond the tile edge themselves),
+// segments of sl rows, four waves per workgroup.  The coefficients live in
+// LDS and are read one box row at a time (81 FP64 values would not fit the
+// registers next to the partial sums).
 #pragma once
 
 #include "engine.hpp"
@@ -62,9 +62,8 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
   __syncthreads();
   if (gw >= a.ntk * a.nseg) return;
   const int tk = gw % a.ntk, seg = gw / a.ntk;
-  const int kv0 = R + tk * a.SK;
-  const int kv1 = min(kv0 + a.SK, a.D2 - R);
-  const int kl = kv0 - R + lane * CK;
+  const int kv0 = R + tk * a.SK;  // the tile's first cell: SK = 64 * CK, no overlap
+  const int kl = kv0 + lane * CK;
   const int i0 = R + seg * a.sl;
   const int sle = min(a.sl, a.D0 - R - i0);
   const int nit = sle + 2 * R;
@@ -75,10 +74,20 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
 #pragma unroll
   for (int kk = 0; kk < CK; kk++) {
     int k = kl + kk;
-    kst[kk] = k >= kv0 && k < kv1;
+    kst[kk] = k < a.D2 - R;
     kst_all = kst_all && kst[kk];
-    k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+    k = k >= a.D2 ? a.D2 - 1 : k;
     vclamp[kk] = (unsigned)k * SZ;
+  }
+  // beyond the tile edge: lane 0 reads the R cells left of kv0, lane 63 the
+  // R cells right of kv0 + SK; the other lanes read their own first cell
+  // (unused)
+  unsigned vedge[R];
+#pragma unroll
+  for (int m = 0; m < R; m++) {
+    int k = lane == 0 ? kv0 - R + m : (lane == 63 ? kv0 + a.SK + m : kl);
+    k = k < 0 ? 0 : (k >= a.D2 ? a.D2 - 1 : k);
+    vedge[m] = (unsigned)k * SZ;
   }
   const T div = a.coef[NT];
   const T rc = a.coef[NT + 1];
@@ -91,27 +100,33 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
 #pragma unroll
     for (int kk = 0; kk < CK; kk++) acc[s][kk] = T(0);
 
-  auto load_row = [&](T (&x)[CK], int n) {
+  auto load_row = [&](T (&x)[CK], T (&e)[R], int n) {
     int r = i0 - R + n;
     r = r < 0 ? 0 : (r >= a.D0 ? a.D0 - 1 : r);
     const auto rs = make_rsrc(a.src + (long long)r * a.s0);
 #pragma unroll
     for (int kk = 0; kk < CK; kk++) x[kk] = bload<false, T>(rs, vclamp[kk], 0);
+#pragma unroll
+    for (int m = 0; m < R; m++) e[m] = bload<false, T>(rs, vedge[m], 0);
   };
-  T nxt[CK];
-  load_row(nxt, 0);
+  T nxt[CK], nxe[R];
+  load_row(nxt, nxe, 0);
   for (int n = 0; n < nit; n++) {
-    T x[CK];
+    T x[CK], e[R];
 #pragma unroll
     for (int kk = 0; kk < CK; kk++) x[kk] = nxt[kk];
-    load_row(nxt, n + 1);  // one row ahead (the last is a clamped stand-in)
+#pragma unroll
+    for (int m = 0; m < R; m++) e[m] = nxe[m];
+    load_row(nxt, nxe, n + 1);  // one row ahead (the last is a clamped stand-in)
     T xe[CK + 2 * R];
 #pragma unroll
     for (int kk = 0; kk < CK; kk++) xe[kk + R] = x[kk];
 #pragma unroll
     for (int m = 1; m <= R; m++) {
-      xe[R - m] = from_left(x[CK - m]);
-      xe[R + CK - 1 + m] = from_right(x[m - 1]);
+      const T l = from_left(x[CK - m]);
+      const T rr = from_right(x[m - 1]);
+      xe[R - m] = lane == 0 ? e[R - m] : l;
+      xe[R + CK - 1 + m] = lane == 63 ? e[m - 1] : rr;
     }
     // box row DI's coefficients are in crb[DI & 1]; row DI + 1's are read
     // while row DI is summed.  Their offsets pass through an empty asm, so

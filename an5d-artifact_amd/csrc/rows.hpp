@@ -8,7 +8,9 @@
 // once and, extended by DPP, adds its 2R+1 row-convolutions into the partial
 // sums of the 2R+1 output rows it touches; an output row is complete (and
 // stored) after its last input row.  Registers: (2R+1) x CK partial sums and
-// one extended row, instead of (2R+1) extended rows per stage.
+// one extended row, instead of (2R+1) extended rows per stage.  The row loop
+// is unrolled by 2R+1, so each output row keeps one register slot from its
+// first input row to its store (no moves between slots).
 //
 // Exactness: the sources list a box's taps row-major (d0, then d2, both
 // ascending: box2d4r.c:14-108), and the input rows arrive in ascending d0, so
@@ -92,8 +94,7 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
   const T div = a.coef[NT];
   const T rc = a.coef[NT + 1];
 
-  // acc[d] holds output m = n - d (input row n = the one being added): box
-  // row d of it arrives now; acc[P-1] completes with this row
+  // partial sums of the P output rows in flight, one slot per row m mod P
   T acc[P][CK];
 #pragma unroll
   for (int s = 0; s < P; s++)
@@ -111,89 +112,94 @@ __device__ __forceinline__ void rows_body(const SweepArgs<T>& a) {
   };
   T nxt[CK], nxe[R];
   load_row(nxt, nxe, 0);
-  for (int n = 0; n < nit; n++) {
-    T x[CK], e[R];
+  // P rows per trip: input row n = n0 + U adds box row DI into the sums of
+  // output n - DI, held in acc[(U - DI) mod P] (n0 is a multiple of P), so the
+  // sums never move between registers
+  for (int n0 = 0; n0 < nit; n0 += P) {
+    static_for<0, P>([&](auto uc) {
+      constexpr int U = decltype(uc)::value;
+      const int n = n0 + U;
+      if (n < nit) {
+        T x[CK], e[R];
 #pragma unroll
-    for (int kk = 0; kk < CK; kk++) x[kk] = nxt[kk];
+        for (int kk = 0; kk < CK; kk++) x[kk] = nxt[kk];
 #pragma unroll
-    for (int m = 0; m < R; m++) e[m] = nxe[m];
-    load_row(nxt, nxe, n + 1);  // one row ahead (the last is a clamped stand-in)
-    T xe[CK + 2 * R];
+        for (int m = 0; m < R; m++) e[m] = nxe[m];
+        load_row(nxt, nxe, n + 1);  // one row ahead (the last is a clamped stand-in)
+        T xe[CK + 2 * R];
 #pragma unroll
-    for (int kk = 0; kk < CK; kk++) xe[kk + R] = x[kk];
+        for (int kk = 0; kk < CK; kk++) xe[kk + R] = x[kk];
 #pragma unroll
-    for (int m = 1; m <= R; m++) {
-      const T l = from_left(x[CK - m]);
-      const T rr = from_right(x[m - 1]);
-      xe[R - m] = lane == 0 ? e[R - m] : l;
-      xe[R + CK - 1 + m] = lane == 63 ? e[m - 1] : rr;
-    }
-    // box row DI's coefficients are in crb[DI & 1]; row DI + 1's are read
-    // while row DI is summed.  Their offsets pass through an empty asm, so
-    // the reads stay in the loop (81 FP64 values would not fit next to the
-    // partial sums) and none is issued before the previous row's sums.
-    T crb[2][P];
-    auto read_coef = [&](T (&cr)[P], int di) {
-      int off = di * P;
-      asm volatile("" : "+s"(off));
+        for (int m = 1; m <= R; m++) {
+          const T l = from_left(x[CK - m]);
+          const T rr = from_right(x[m - 1]);
+          xe[R - m] = lane == 0 ? e[R - m] : l;
+          xe[R + CK - 1 + m] = lane == 63 ? e[m - 1] : rr;
+        }
+        // box row DI's coefficients are in crb[DI & 1]; row DI + 1's are read
+        // while row DI is summed.  Their offsets pass through an empty asm, so
+        // the reads stay in the loop (81 FP64 values would not fit next to the
+        // partial sums) and none is issued before the previous row's sums.
+        T crb[2][P];
+        auto read_coef = [&](T (&cr)[P], int di) {
+          int off = di * P;
+          asm volatile("" : "+s"(off));
 #pragma unroll
-      for (int d = 0; d < P; d++) cr[d] = cs[off + d];
-    };
-    read_coef(crb[0], 0);
-    static_for<0, P>([&](auto dc) {
-      constexpr int DI = decltype(dc)::value;
-      if constexpr (DI + 1 < P) read_coef(crb[(DI + 1) & 1], DI + 1);
-      const T(&cr)[P] = crb[DI & 1];
-      // tap by tap over the CK cells: each cell's sum keeps the source order
-      static_for<0, P>([&](auto ec) {
-        constexpr int E = decltype(ec)::value;
-        constexpr TapDef t = D::taps[DI * P + E];
+          for (int d = 0; d < P; d++) cr[d] = cs[off + d];
+        };
+        read_coef(crb[0], 0);
+        static_for<0, P>([&](auto dc) {
+          constexpr int DI = decltype(dc)::value;
+          if constexpr (DI + 1 < P) read_coef(crb[(DI + 1) & 1], DI + 1);
+          const T(&cr)[P] = crb[DI & 1];
+          // tap by tap over the CK cells: each cell's sum keeps the source order
+          static_for<0, P>([&](auto ec) {
+            constexpr int E = decltype(ec)::value;
+            constexpr TapDef t = D::taps[DI * P + E];
 #pragma unroll
-        for (int kk = 0; kk < CK; kk++) {
-          T& s = acc[DI][kk];
-          const T v = xe[kk + E];
-          if constexpr (EXACT) {
-            const T prod = cr[E] * v;
-            if constexpr (DI == 0 && E == 0) s = t.sign > 0 ? prod : -prod;
-            else if constexpr (t.sign > 0) s = s + prod;
-            else s = s - prod;
+            for (int kk = 0; kk < CK; kk++) {
+              T& s = acc[(U - DI + P) % P][kk];
+              const T v = xe[kk + E];
+              if constexpr (EXACT) {
+                const T prod = cr[E] * v;
+                if constexpr (DI == 0 && E == 0) s = t.sign > 0 ? prod : -prod;
+                else if constexpr (t.sign > 0) s = s + prod;
+                else s = s - prod;
+              } else {
+                if constexpr (DI == 0 && E == 0) s = cr[E] * v;
+                else s = fma_t(cr[E], v, s);
+              }
+            }
+          });
+          // the sums of this box row are formed here: without this the compiler
+          // sinks them below the store branch, and the reads of every box row's
+          // coefficients then sit together at the top of the loop (81 values
+          // live at once: for box2d4r FP64, 284 registers and one wave per SIMD
+          // instead of about 150 and three)
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) asm volatile("" : "+v"(acc[(U - DI + P) % P][kk]));
+        });
+        // output m = n - 2R (slot (U + 1) mod P) received its last row
+        const int m = n - 2 * R;
+        if (m >= 0) {
+          T out[CK];
+#pragma unroll
+          for (int kk = 0; kk < CK; kk++) {
+            if constexpr (DM > 0) out[kk] = div_by(acc[(U + 1) % P][kk], div, rc, DM);
+            else out[kk] = acc[(U + 1) % P][kk];
+          }
+          T* row = a.dst + (long long)(i0 + m) * a.s0 + kl;
+          if (kst_all) {
+            if (a.nt_store) gstore_run<true, T, CK>(row, out);
+            else gstore_run<false, T, CK>(row, out);
           } else {
-            if constexpr (DI == 0 && E == 0) s = cr[E] * v;
-            else s = fma_t(cr[E], v, s);
+#pragma unroll
+            for (int kk = 0; kk < CK; kk++)
+              if (kst[kk]) row[kk] = out[kk];
           }
         }
-      });
-      // the sums of this box row are formed here: without this the compiler
-      // sinks them below the store branch, and the reads of every box row's
-      // coefficients then sit together at the top of the loop (81 values
-      // live at once: for box2d4r FP64, 284 registers and one wave per SIMD
-      // instead of about 150 and three)
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) asm volatile("" : "+v"(acc[DI][kk]));
+      }
     });
-    // output m = n - 2R received its last row
-    const int m = n - 2 * R;
-    if (m >= 0) {
-      T out[CK];
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) {
-        if constexpr (DM > 0) out[kk] = div_by(acc[P - 1][kk], div, rc, DM);
-        else out[kk] = acc[P - 1][kk];
-      }
-      T* row = a.dst + (long long)(i0 + m) * a.s0 + kl;
-      if (kst_all) {
-        if (a.nt_store) gstore_run<true, T, CK>(row, out);
-        else gstore_run<false, T, CK>(row, out);
-      } else {
-#pragma unroll
-        for (int kk = 0; kk < CK; kk++)
-          if (kst[kk]) row[kk] = out[kk];
-      }
-    }
-#pragma unroll
-    for (int d = P - 1; d > 0; d--)
-#pragma unroll
-      for (int kk = 0; kk < CK; kk++) acc[d][kk] = acc[d - 1][kk];
   }
 }
 
